@@ -1,0 +1,798 @@
+// C-ABI (include/ddg.h) over the HIP kernels: device memory ownership,
+// launch sizing, the batch_insert driver and error reporting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ddg.h"
+#include "device.hpp"
+#include "internal.hpp"
+
+using namespace ddg;
+
+namespace {
+
+constexpr uint32_t kLdsPerCU = 160u * 1024u;
+
+struct Buf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+struct Scratch {
+  Buf b[32];
+};
+
+enum BufId {
+  B_QUERIES, B_STARTS, B_QBEAM, B_QIDS, B_OUT_IDS, B_OUT_D, B_CMPS, B_VISN, B_VIS_IDS, B_VIS_D, B_FLAGS,
+  B_INSERTS, B_NEW_IDS, B_NEW_LEN, B_OFFS, B_KEYS, B_VALS, B_KEYS2, B_VALS2, B_TMP, B_GFLAGS, B_GOFFS, B_GKEYS,
+  B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D
+};
+
+}  // namespace
+
+struct ddg_index {
+  int dtype = 0, metric = 0;
+  uint64_t n = 0;
+  uint32_t d = 0, ad = 0, R = 0, es = 1;
+  void* rows = nullptr;
+  uint32_t* graph = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t num_cu = 256;
+  Scratch s;
+  std::string err;
+  double prof[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+static std::string g_err;
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail(idx, DDG_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+static ddg_status fail(ddg_index* idx, ddg_status st, const std::string& msg) {
+  if (idx) idx->err = msg; else g_err = msg;
+  return st;
+}
+
+static hipError_t ensure(ddg_index* idx, int id, size_t bytes) {
+  Buf& b = idx->s.b[id];
+  if (b.cap >= bytes && b.p) return hipSuccess;
+  if (b.p) { (void)hipFree(b.p); b.p = nullptr; b.cap = 0; }
+  size_t want = std::max<size_t>(bytes, 256);
+  hipError_t e = hipMalloc(&b.p, want);
+  if (e != hipSuccess) { b.p = nullptr; return e; }
+  b.cap = want;
+  return hipSuccess;
+}
+
+template <class T>
+static T* buf(ddg_index* idx, int id) { return (T*)idx->s.b[id].p; }
+
+#define ENSURE(id, bytes)                                                                            \
+  do {                                                                                               \
+    hipError_t e_ = ensure(idx, (id), (bytes));                                                      \
+    if (e_ != hipSuccess) return fail(idx, e_ == hipErrorOutOfMemory ? DDG_EOOM : DDG_EDEVICE,       \
+                                      std::string("device allocation failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static uint32_t round_up(uint32_t x, uint32_t m) { return (x + m - 1) / m * m; }
+
+// ----------------------------------------------------------- launch sizing
+struct SearchLayout {
+  SearchArgs a;
+  Launch l;
+  bool ok;
+};
+
+static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, SearchArgs& a, Launch& l) {
+  const uint32_t qrow = round_up(idx->ad * idx->es, 16);
+  const uint32_t bits = filter_bits(beam_max);
+  const uint32_t table = 4u << bits;
+  const uint32_t fcap = round_up(std::max(beam_max, nstarts), 16);
+  const uint32_t rcap = round_up(std::max<uint32_t>(idx->R, 64), 64);
+  uint32_t off = 0;
+  off += qrow;
+  a.off_table = off; off += table;
+  a.off_f0 = off; off += fcap * 8;
+  a.off_f1 = off; off += fcap * 8;
+  a.off_keep = off; off += rcap * 4;
+  a.off_cand = off; off += rcap * 8;
+  a.off_sorted = off; off += rcap * 8;
+  a.lds_per_wave = round_up(off, 16);
+  a.fcap = fcap;
+  a.rcap = rcap;
+  if (a.lds_per_wave > kLdsPerCU) return false;
+  uint32_t wpb = std::min<uint32_t>(4, kLdsPerCU / a.lds_per_wave);
+  const uint32_t per_block = wpb * a.lds_per_wave;
+  uint32_t bpc = std::max<uint32_t>(1, std::min<uint32_t>(32 / wpb, kLdsPerCU / per_block));
+  l.waves_per_block = wpb;
+  l.lds_bytes = per_block;
+  l.blocks = idx->num_cu * bpc;
+  return true;
+}
+
+static void prune_layout(const ddg_index* idx, uint32_t Rsel, uint32_t want_ccap, PruneArgs& a, Launch& l) {
+  const uint32_t row = round_up(idx->ad * idx->es, 16);
+  uint32_t ccap = 64;
+  while (ccap < want_ccap && ccap < 4096) ccap <<= 1;
+  auto per_wave = [&](uint32_t c) { return 2 * row + round_up(Rsel * 4, 16) + c * 8; };
+  while (ccap > 64 && per_wave(ccap) * 2 > kLdsPerCU) ccap >>= 1;
+  a.ccap = ccap;
+  a.off_cand = 2 * row + round_up(Rsel * 4, 16);
+  a.lds_per_wave = round_up(per_wave(ccap), 16);
+  uint32_t wpb = std::min<uint32_t>(4, kLdsPerCU / a.lds_per_wave);
+  const uint32_t per_block = wpb * a.lds_per_wave;
+  uint32_t bpc = std::max<uint32_t>(1, std::min<uint32_t>(32 / wpb, kLdsPerCU / per_block));
+  l.waves_per_block = wpb;
+  l.lds_bytes = per_block;
+  l.blocks = idx->num_cu * bpc;
+}
+
+// ---------------------------------------------------------------- timing
+struct Timer {
+  ddg_index* idx;
+  double* acc;
+  Timer(ddg_index* i, double* a) : idx(i), acc(a) { (void)hipEventRecord(idx->ev0, idx->stream); }
+  void stop() {
+    (void)hipEventRecord(idx->ev1, idx->stream);
+    (void)hipEventSynchronize(idx->ev1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, idx->ev0, idx->ev1);
+    *acc += ms;
+  }
+};
+
+// ------------------------------------------------------------- build parts
+static uint32_t key_bits_for(uint64_t n) {
+  uint32_t b = 1;
+  while (b < 32 && (1ull << b) < n) b++;
+  return ((b + 7) / 8) * 8;
+}
+
+// beam search in build mode over pts[0..B) (device), visited lists to
+// B_VIS_IDS / B_VIS_D with stride *vcap, lengths to B_VISN.
+static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_t B, uint32_t L, uint32_t start,
+                                    uint32_t* vcap, uint64_t* cmps_total) {
+  for (int attempt = 0; attempt < 8; attempt++) {
+    ENSURE(B_VIS_IDS, (size_t)B * *vcap * 4);
+    ENSURE(B_VIS_D, (size_t)B * *vcap * 4);
+    ENSURE(B_VISN, (size_t)B * 4);
+    ENSURE(B_CMPS, (size_t)B * 8);
+    ENSURE(B_FLAGS, 64);
+    ENSURE(B_STARTS, 64);
+    HIPCHK(hipMemcpyAsync(buf<uint32_t>(idx, B_STARTS), &start, 4, hipMemcpyHostToDevice, idx->stream));
+    uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);  // [0] work, [1] overflow
+    HIPCHK(launch_fill_u32(flags, 0u, 16, idx->stream));
+    SearchArgs a{};
+    Launch l{};
+    if (!search_layout(idx, L, 1, a, l)) return fail(idx, DDG_EINVAL, "build beam L too large for LDS");
+    a.base = idx->rows;
+    a.graph = idx->graph;
+    a.qids = pts;
+    a.starts = buf<uint32_t>(idx, B_STARTS);
+    a.dist_cmps = buf<uint64_t>(idx, B_CMPS);
+    a.visited_count = buf<uint32_t>(idx, B_VISN);
+    a.vis_ids = buf<uint32_t>(idx, B_VIS_IDS);
+    a.vis_dists = buf<float>(idx, B_VIS_D);
+    a.overflow = flags + 1;
+    a.work = flags;
+    a.n = idx->n;
+    a.nq = B;
+    a.limit = idx->n;
+    a.cut = 0.0;
+    a.d = idx->d;
+    a.ad = idx->ad;
+    a.R = idx->R;
+    a.nstarts = 1;
+    a.k = 0;
+    a.beam = L;
+    a.degree_limit = idx->R;
+    a.vcap = *vcap;
+    a.build_mode = 1;
+    a.cut_trim = 0;
+    l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (B + l.waves_per_block - 1) / l.waves_per_block);
+    Timer t(idx, &idx->prof[0]);
+    HIPCHK(launch_search(idx->dtype, idx->metric, a, l, idx->stream));
+    t.stop();
+    uint32_t ov = 0;
+    HIPCHK(hipMemcpy(&ov, flags + 1, 4, hipMemcpyDeviceToHost));
+    if (!ov) {
+      if (cmps_total) {
+        std::vector<uint64_t> c(B);
+        HIPCHK(hipMemcpy(c.data(), buf<uint64_t>(idx, B_CMPS), B * 8, hipMemcpyDeviceToHost));
+        for (uint64_t x : c) *cmps_total += x;
+      }
+      return DDG_OK;
+    }
+    *vcap *= 2;  // a visited list outgrew its slot: rerun the batch with room to spare
+  }
+  return fail(idx, DDG_EINVAL, "visited lists exceed the capacity limit");
+}
+
+// robustPrune launch over np tasks; candidates either fixed-stride lists or CSR.
+static ddg_status run_prune(ddg_index* idx, PruneArgs a, uint64_t np, uint32_t max_cand, double* acc) {
+  if (np == 0) return DDG_OK;
+  Launch l{};
+  prune_layout(idx, a.Rsel, max_cand, a, l);
+  ENSURE(B_FLAGS, 64);
+  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);  // [2] work, [3] nbig, [4] status, [5] work2
+  HIPCHK(launch_fill_u32(flags + 2, 0u, 4, idx->stream));
+  ENSURE(B_BIG, (size_t)np * 4);
+  a.base = idx->rows;
+  a.graph = idx->graph;
+  a.work = flags + 2;
+  a.nbig = flags + 3;
+  a.status = flags + 4;
+  a.big_tasks = buf<uint32_t>(idx, B_BIG);
+  a.np = np;
+  a.d = idx->d;
+  a.ad = idx->ad;
+  a.R1 = idx->R + 1;
+  a.big_pass = 0;
+  l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (np + l.waves_per_block - 1) / l.waves_per_block);
+  Timer t(idx, acc);
+  HIPCHK(launch_prune(idx->dtype, idx->metric, a, l, idx->stream));
+  t.stop();
+  uint32_t st[2];
+  HIPCHK(hipMemcpy(st, flags + 3, 8, hipMemcpyDeviceToHost));
+  if (st[0]) {
+    // candidate lists beyond the LDS capacity: global scratch, one slot per wave
+    uint32_t gcap = 64;
+    while (gcap < max_cand) gcap <<= 1;
+    const uint32_t waves = 4 * idx->num_cu;
+    ENSURE(B_GSD, (size_t)waves * gcap * 4);
+    ENSURE(B_GSI, (size_t)waves * gcap * 4);
+    a.gscratch_d = buf<float>(idx, B_GSD);
+    a.gscratch_i = buf<uint32_t>(idx, B_GSI);
+    a.gcap = gcap;
+    a.big_pass = 1;
+    a.work = flags + 5;
+    HIPCHK(launch_fill_u32(flags + 5, 0u, 1, idx->stream));
+    Launch lb = l;
+    lb.waves_per_block = 4;
+    lb.blocks = idx->num_cu;
+    lb.lds_bytes = 4 * round_up(2 * round_up(idx->ad * idx->es, 16) + round_up(a.Rsel * 4, 16), 16);
+    a.lds_per_wave = lb.lds_bytes / 4;
+    Timer t2(idx, acc);
+    HIPCHK(launch_prune(idx->dtype, idx->metric, a, lb, idx->stream));
+    t2.stop();
+  }
+  HIPCHK(hipMemcpy(st, flags + 4, 4, hipMemcpyDeviceToHost));
+  if (st[0] & 1u) return fail(idx, DDG_EDEGREE, "adjacency list would exceed max degree");
+  if (st[0] & 2u) return fail(idx, DDG_EINVAL, "prune candidate list exceeds scratch capacity");
+  return DDG_OK;
+}
+
+static void schedule(uint64_t m, double base, size_t max_batch, std::vector<std::pair<uint64_t, uint64_t>>& out) {
+  // index.h:250-261 / bsp_index.hpp:238-246
+  out.clear();
+  size_t inc = 0, count = 0;
+  while (count < m) {
+    size_t lo, hi;
+    if (std::pow(base, (double)inc) <= (double)max_batch) {
+      lo = (size_t)std::pow(base, (double)inc) - 1;
+      hi = std::min<size_t>((size_t)std::pow(base, (double)(inc + 1)), m) - 1;
+      count = hi;
+    } else {
+      lo = count;
+      hi = std::min<size_t>(count + max_batch, m);
+      count += max_batch;
+    }
+    out.push_back({lo, hi});
+    inc++;
+  }
+}
+
+static size_t max_batch_size(double max_fraction, uint64_t count) {
+  // index.h:233-236 / bsp_index.hpp:164-166
+  size_t mb = std::min<size_t>((size_t)(max_fraction * (double)(float)count), 1000000ul);
+  return mb == 0 ? (size_t)count : mb;
+}
+
+__global__ void k_acc_visited(uint32_t* vc, const uint32_t* pts, const uint32_t* visn, uint64_t B) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < B) vc[pts[i]] += visn[i];
+}
+
+__global__ void k_max_group(const uint64_t* goff, const uint64_t* ngroups, uint32_t* out) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < *ngroups) atomicMax(out, (uint32_t)(goff[g + 1] - goff[g]));
+}
+
+// knn_index::batch_insert over device ids ins[0..m)
+static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t m, double alpha, uint32_t L,
+                                   uint32_t Rsel, uint32_t start, double base, double max_fraction,
+                                   uint64_t max_batch_base, uint32_t* vc_dev) {
+  std::vector<std::pair<uint64_t, uint64_t>> batches;
+  schedule(m, base, max_batch_size(max_fraction, max_batch_base), batches);
+  uint32_t vcap = std::max<uint32_t>(2 * L + 64, 128);
+  const uint32_t kb = key_bits_for(idx->n);
+  uint64_t total_cmps = 0;
+  for (auto [lo, hi] : batches) {
+    const uint64_t Bn = hi - lo;
+    if (Bn == 0) continue;
+    const uint32_t* pts = ins + lo;
+    ddg_status st = run_insert_search(idx, pts, Bn, L, start, &vcap, &total_cmps);
+    if (st != DDG_OK) return st;
+    if (vc_dev)
+      hipLaunchKernelGGL(k_acc_visited, dim3((Bn + 255) / 256), dim3(256), 0, idx->stream, vc_dev, pts,
+                         buf<uint32_t>(idx, B_VISN), Bn);
+    // forward prune (index.h:281)
+    ENSURE(B_NEW_IDS, (size_t)Bn * Rsel * 4);
+    ENSURE(B_NEW_LEN, (size_t)Bn * 4);
+    PruneArgs pa{};
+    pa.pts = pts;
+    pa.a_ids = buf<uint32_t>(idx, B_VIS_IDS);
+    pa.a_dists = buf<float>(idx, B_VIS_D);
+    pa.a_len = buf<uint32_t>(idx, B_VISN);
+    pa.a_stride = vcap;
+    pa.out_ids = buf<uint32_t>(idx, B_NEW_IDS);
+    pa.out_len = buf<uint32_t>(idx, B_NEW_LEN);
+    pa.alpha = alpha;
+    pa.Rsel = Rsel;
+    pa.add = 1;
+    st = run_prune(idx, pa, Bn, vcap + idx->R, &idx->prof[1]);
+    if (st != DDG_OK) return st;
+    // graph update + pairs (index.h:287-301)
+    Timer te(idx, &idx->prof[2]);
+    ENSURE(B_OFFS, (size_t)(Bn + 1) * 8);
+    ENSURE(B_TMP, std::max(scan_tmp_bytes(Bn), radix_tmp_bytes((uint64_t)Bn * Rsel)) + 64);
+    ENSURE(B_FLAGS, 64);
+    uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+    HIPCHK(launch_fill_u32(flags + 4, 0u, 1, idx->stream));
+    HIPCHK(launch_apply_rows(idx->graph, idx->R + 1, pts, buf<uint32_t>(idx, B_NEW_IDS), buf<uint32_t>(idx, B_NEW_LEN),
+                             Rsel, Bn, flags + 4, idx->stream));
+    uint64_t* dtotal = (uint64_t*)(flags + 8);
+    HIPCHK(scan_exclusive_u32(buf<uint32_t>(idx, B_NEW_LEN), buf<uint64_t>(idx, B_OFFS), Bn, dtotal,
+                              buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap, idx->stream));
+    uint64_t P = 0;
+    HIPCHK(hipMemcpyAsync(&P, dtotal, 8, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    if (P == 0) { te.stop(); continue; }
+    ENSURE(B_KEYS, P * 4);
+    ENSURE(B_VALS, P * 4);
+    ENSURE(B_KEYS2, P * 4);
+    ENSURE(B_VALS2, P * 4);
+    ENSURE(B_TMP, std::max(scan_tmp_bytes(P), radix_tmp_bytes(P)) + 64);
+    HIPCHK(launch_emit_pairs(pts, buf<uint32_t>(idx, B_NEW_IDS), buf<uint32_t>(idx, B_NEW_LEN), buf<uint64_t>(idx, B_OFFS),
+                             Rsel, Bn, buf<uint32_t>(idx, B_KEYS), buf<uint32_t>(idx, B_VALS), idx->stream));
+    bool in_alt = false;
+    HIPCHK(radix_sort_pairs(buf<uint32_t>(idx, B_KEYS), buf<uint32_t>(idx, B_VALS), buf<uint32_t>(idx, B_KEYS2),
+                            buf<uint32_t>(idx, B_VALS2), P, kb, buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap, idx->stream,
+                            &in_alt));
+    uint32_t* skeys = in_alt ? buf<uint32_t>(idx, B_KEYS2) : buf<uint32_t>(idx, B_KEYS);
+    uint32_t* svals = in_alt ? buf<uint32_t>(idx, B_VALS2) : buf<uint32_t>(idx, B_VALS);
+    ENSURE(B_GFLAGS, P * 4);
+    ENSURE(B_GOFFS, P * 8);
+    ENSURE(B_GKEYS, P * 4);
+    ENSURE(B_GOFF, (P + 1) * 8);
+    uint64_t* dng = (uint64_t*)(flags + 10);
+    HIPCHK(launch_group_starts(skeys, P, buf<uint32_t>(idx, B_GFLAGS), buf<uint64_t>(idx, B_GOFFS), dng,
+                               buf<uint32_t>(idx, B_GKEYS), buf<uint64_t>(idx, B_GOFF), buf<void>(idx, B_TMP),
+                               idx->s.b[B_TMP].cap, idx->stream));
+    HIPCHK(launch_fill_u32(flags + 12, 0u, 1, idx->stream));
+    hipLaunchKernelGGL(k_max_group, dim3((P + 255) / 256), dim3(256), 0, idx->stream, buf<uint64_t>(idx, B_GOFF), dng,
+                       flags + 12);
+    uint32_t hostf[9];
+    HIPCHK(hipMemcpyAsync(hostf, flags + 4, 9 * 4, hipMemcpyDeviceToHost, idx->stream));
+    HIPCHK(hipStreamSynchronize(idx->stream));
+    te.stop();
+    if (hostf[0] & 1u) return fail(idx, DDG_EDEGREE, "update_neighbors: cannot exceed max degree");
+    uint64_t ngroups = 0;
+    std::memcpy(&ngroups, &hostf[6], 8);
+    const uint32_t maxg = hostf[8];
+    // reverse edges: append or prune per target (index.h:309-322)
+    PruneArgs ra{};
+    ra.pts = buf<uint32_t>(idx, B_GKEYS);
+    ra.a_ids = svals;
+    ra.a_off = buf<uint64_t>(idx, B_GOFF);
+    ra.alpha = alpha;
+    ra.Rsel = Rsel;
+    ra.add = 1;
+    ra.write_graph = 1;
+    ra.append_if_fits = 1;
+    st = run_prune(idx, ra, ngroups, maxg + idx->R, &idx->prof[3]);
+    if (st != DDG_OK) return st;
+  }
+  idx->prof[7] += (double)total_cmps;
+  return DDG_OK;
+}
+
+// ====================================================================== ABI
+extern "C" {
+
+const char* ddg_last_error(const ddg_index* idx) { return idx ? idx->err.c_str() : g_err.c_str(); }
+
+ddg_status ddg_create(ddg_dtype dtype, ddg_metric metric, uint64_t n, uint32_t d, uint32_t aligned_d,
+                      const void* rows, uint32_t R, ddg_index** out) {
+  ddg_index* idx = nullptr;
+  if (!out) return fail(idx, DDG_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (dtype < DDG_U8 || dtype > DDG_F32 || metric < DDG_L2 || metric > DDG_MIPS)
+    return fail(idx, DDG_EINVAL, "bad dtype/metric");
+  if (n == 0 || d == 0 || aligned_d < d || !rows || R == 0) return fail(idx, DDG_EINVAL, "bad sizes");
+  if (n >= (1ull << 31)) return fail(idx, DDG_EINVAL, "n must be below 2^31");
+  const uint32_t es = dtype == DDG_F32 ? 4 : 1;
+  if ((aligned_d * es) % 16 != 0) return fail(idx, DDG_EINVAL, "aligned_d * sizeof(T) must be a multiple of 16");
+  if (d * es > 16384) return fail(idx, DDG_EINVAL, "rows above 16 KiB are not supported");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(idx, DDG_EDEVICE, "no HIP device");
+  idx = new ddg_index();
+  idx->dtype = dtype;
+  idx->metric = metric;
+  idx->n = n;
+  idx->d = d;
+  idx->ad = aligned_d;
+  idx->R = R;
+  idx->es = es;
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+    idx->num_cu = (uint32_t)prop.multiProcessorCount;
+  ddg_status st = DDG_OK;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && st == DDG_OK) {
+      st = e == hipErrorOutOfMemory ? DDG_EOOM : DDG_EDEVICE;
+      idx->err = std::string(what) + ": " + hipGetErrorString(e);
+    }
+  };
+  chk(hipStreamCreate(&idx->stream), "hipStreamCreate");
+  chk(hipEventCreate(&idx->ev0), "hipEventCreate");
+  chk(hipEventCreate(&idx->ev1), "hipEventCreate");
+  const size_t row_bytes = (size_t)aligned_d * es;
+  chk(hipMalloc(&idx->rows, n * row_bytes), "hipMalloc rows");
+  chk(hipMalloc((void**)&idx->graph, n * (size_t)(R + 1) * 4), "hipMalloc graph");
+  if (st == DDG_OK) chk(hipMemcpyAsync(idx->rows, rows, n * row_bytes, hipMemcpyHostToDevice, idx->stream), "H2D rows");
+  if (st == DDG_OK) chk(launch_pad_rows(idx->rows, n, d * es, (uint32_t)row_bytes, idx->stream), "pad rows");
+  if (st == DDG_OK) chk(hipMemsetAsync(idx->graph, 0, n * (size_t)(R + 1) * 4, idx->stream), "zero graph");
+  if (st == DDG_OK) chk(hipStreamSynchronize(idx->stream), "sync");
+  if (st != DDG_OK) {
+    g_err = idx->err;
+    ddg_destroy(idx);
+    return st;
+  }
+  *out = idx;
+  return DDG_OK;
+}
+
+void ddg_destroy(ddg_index* idx) {
+  if (!idx) return;
+  if (idx->stream) hipStreamSynchronize(idx->stream);
+  for (auto& b : idx->s.b)
+    if (b.p) hipFree(b.p);
+  if (idx->rows) hipFree(idx->rows);
+  if (idx->graph) hipFree(idx->graph);
+  if (idx->ev0) hipEventDestroy(idx->ev0);
+  if (idx->ev1) hipEventDestroy(idx->ev1);
+  if (idx->stream) hipStreamDestroy(idx->stream);
+  delete idx;
+}
+
+uint64_t ddg_size(const ddg_index* idx) { return idx ? idx->n : 0; }
+uint32_t ddg_max_degree(const ddg_index* idx) { return idx ? idx->R : 0; }
+uint32_t ddg_dims(const ddg_index* idx) { return idx ? idx->d : 0; }
+uint32_t ddg_aligned_dims(const ddg_index* idx) { return idx ? idx->ad : 0; }
+
+ddg_status ddg_last_profile(const ddg_index* idx, double* out9) {
+  if (!idx || !out9) return DDG_EINVAL;
+  for (int i = 0; i < 9; i++) out9[i] = idx->prof[i];
+  return DDG_OK;
+}
+
+ddg_status ddg_graph_set(ddg_index* idx, const uint32_t* rows) {
+  if (!idx || !rows) return fail(idx, DDG_EINVAL, "NULL argument");
+  for (uint64_t i = 0; i < idx->n; i++) {
+    const uint32_t* r = rows + i * (uint64_t)(idx->R + 1);
+    if (r[0] > idx->R) return fail(idx, DDG_EDEGREE, "graph row degree exceeds R");
+    for (uint32_t j = 0; j < r[0]; j++)
+      if (r[1 + j] >= idx->n) return fail(idx, DDG_ERANGE, "graph row holds an id >= n");
+  }
+  HIPCHK(hipMemcpyAsync(idx->graph, rows, idx->n * (size_t)(idx->R + 1) * 4, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(hipStreamSynchronize(idx->stream));
+  return DDG_OK;
+}
+
+ddg_status ddg_graph_get(ddg_index* idx, uint32_t* rows) {
+  if (!idx || !rows) return fail(idx, DDG_EINVAL, "NULL argument");
+  HIPCHK(hipMemcpyAsync(rows, idx->graph, idx->n * (size_t)(idx->R + 1) * 4, hipMemcpyDeviceToHost, idx->stream));
+  HIPCHK(hipStreamSynchronize(idx->stream));
+  return DDG_OK;
+}
+
+static ddg_status build_common(ddg_index* idx, const ddg_build_params* bp, uint32_t* visited_counts,
+                               double* build_seconds) {
+  if (!bp) return fail(idx, DDG_EINVAL, "NULL params");
+  if (bp->R == 0 || bp->R > idx->R) return fail(idx, DDG_EINVAL, "BuildParams.R must be in [1, graph R]");
+  if (bp->L == 0) return fail(idx, DDG_EINVAL, "BuildParams.L must be positive");
+  if (bp->start_point >= idx->n) return fail(idx, DDG_ERANGE, "start point out of range");
+  if (!(bp->base > 1.0)) return fail(idx, DDG_EINVAL, "base must exceed 1");
+  for (double& p : idx->prof) p = 0;
+  const uint64_t n = idx->n;
+  uint64_t m = n;
+  if (bp->bsp && bp->max_build) m = std::min<uint64_t>(bp->max_build, n);
+  ENSURE(B_INSERTS, n * 4);
+  std::vector<uint32_t> ids(n);
+  for (uint64_t i = 0; i < n; i++) ids[i] = (uint32_t)i;
+  HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_INSERTS), ids.data(), n * 4, hipMemcpyHostToDevice));
+  uint32_t* vc = nullptr;
+  if (visited_counts) {
+    ENSURE(B_VC, n * 4);
+    vc = buf<uint32_t>(idx, B_VC);
+    HIPCHK(launch_fill_u32(vc, 0u, n, idx->stream));
+  }
+  HIPCHK(hipStreamSynchronize(idx->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, idx->stream));
+  const uint64_t mbb = bp->bsp ? m : n;
+  ddg_status st = DDG_OK;
+  if (bp->two_pass && !bp->bsp)
+    st = batch_insert_dev(idx, buf<uint32_t>(idx, B_INSERTS), m, 1.0, bp->L, bp->R, bp->start_point, bp->base,
+                          bp->max_fraction, mbb, vc);
+  if (st == DDG_OK)
+    st = batch_insert_dev(idx, buf<uint32_t>(idx, B_INSERTS), m, bp->alpha, bp->L, bp->R, bp->start_point, bp->base,
+                          bp->max_fraction, mbb, vc);
+  if (st == DDG_OK && bp->final_sort) {
+    Timer t(idx, &idx->prof[4]);
+    HIPCHK(launch_sort_rows(idx->dtype, idx->metric, idx->rows, idx->graph, n, idx->R + 1, idx->d, idx->ad,
+                            idx->stream));
+    t.stop();
+  }
+  HIPCHK(hipEventRecord(e1, idx->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (build_seconds) *build_seconds = ms / 1000.0;
+  if (st != DDG_OK) return st;
+  if (visited_counts) HIPCHK(hipMemcpy(visited_counts, vc, n * 4, hipMemcpyDeviceToHost));
+  return DDG_OK;
+}
+
+ddg_status ddg_build(ddg_index* idx, const ddg_build_params* bp, uint32_t* visited_counts, double* build_seconds) {
+  if (!idx) return fail(idx, DDG_EINVAL, "NULL index");
+  return build_common(idx, bp, visited_counts, build_seconds);
+}
+
+ddg_status ddg_batch_insert(ddg_index* idx, const uint32_t* inserts, uint64_t m, double alpha, uint32_t L,
+                            uint32_t start_point, int random_order, double base, double max_fraction) {
+  if (!idx || (!inserts && m)) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (random_order) return fail(idx, DDG_EINVAL, "random_order is not supported (deterministic order only)");
+  if (L == 0 || !(base > 1.0)) return fail(idx, DDG_EINVAL, "bad L/base");
+  if (start_point >= idx->n) return fail(idx, DDG_ERANGE, "start point out of range");
+  for (uint64_t i = 0; i < m; i++)
+    if (inserts[i] >= idx->n) return fail(idx, DDG_ERANGE, "ERROR: invalid point given to batch_insert");
+  for (double& p : idx->prof) p = 0;
+  ENSURE(B_INSERTS, std::max<uint64_t>(m, 1) * 4);
+  HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_INSERTS), inserts, m * 4, hipMemcpyHostToDevice));
+  return batch_insert_dev(idx, buf<uint32_t>(idx, B_INSERTS), m, alpha, L, idx->R, start_point, base, max_fraction,
+                          idx->n, nullptr);
+}
+
+ddg_status ddg_sort_rows(ddg_index* idx) {
+  if (!idx) return fail(idx, DDG_EINVAL, "NULL index");
+  HIPCHK(launch_sort_rows(idx->dtype, idx->metric, idx->rows, idx->graph, idx->n, idx->R + 1, idx->d, idx->ad,
+                          idx->stream));
+  HIPCHK(hipStreamSynchronize(idx->stream));
+  return DDG_OK;
+}
+
+ddg_status ddg_robust_prune(ddg_index* idx, uint64_t np, const uint32_t* pts, const uint64_t* cand_off,
+                            const uint32_t* cand_ids, const float* cand_dists, double alpha, int add, uint32_t R,
+                            uint32_t* out_ids, uint32_t* out_len) {
+  if (!idx || (np && (!pts || !cand_off || !out_ids || !out_len))) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (R == 0) return fail(idx, DDG_EINVAL, "R must be positive");
+  if (np == 0) return DDG_OK;
+  const uint64_t nc = cand_off[np];
+  uint32_t maxc = 0;
+  for (uint64_t t = 0; t < np; t++) {
+    if (pts[t] >= idx->n) return fail(idx, DDG_ERANGE, "prune point out of range");
+    if (cand_off[t + 1] < cand_off[t]) return fail(idx, DDG_EINVAL, "cand_off must be non-decreasing");
+    maxc = std::max<uint32_t>(maxc, (uint32_t)(cand_off[t + 1] - cand_off[t]));
+  }
+  for (uint64_t i = 0; i < nc; i++)
+    if (cand_ids[i] >= idx->n) return fail(idx, DDG_ERANGE, "candidate id out of range");
+  ENSURE(B_QIDS, np * 4);
+  ENSURE(B_PRUNE_OFF, (np + 1) * 8);
+  ENSURE(B_PRUNE_IDS, std::max<uint64_t>(nc, 1) * 4);
+  ENSURE(B_PRUNE_D, std::max<uint64_t>(nc, 1) * 4);
+  ENSURE(B_NEW_IDS, np * R * 4);
+  ENSURE(B_NEW_LEN, np * 4);
+  HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_QIDS), pts, np * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(buf<uint64_t>(idx, B_PRUNE_OFF), cand_off, (np + 1) * 8, hipMemcpyHostToDevice));
+  if (nc) HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_PRUNE_IDS), cand_ids, nc * 4, hipMemcpyHostToDevice));
+  if (nc && cand_dists) HIPCHK(hipMemcpy(buf<float>(idx, B_PRUNE_D), cand_dists, nc * 4, hipMemcpyHostToDevice));
+  PruneArgs a{};
+  a.pts = buf<uint32_t>(idx, B_QIDS);
+  a.a_ids = buf<uint32_t>(idx, B_PRUNE_IDS);
+  a.a_dists = cand_dists ? buf<float>(idx, B_PRUNE_D) : nullptr;
+  a.a_off = buf<uint64_t>(idx, B_PRUNE_OFF);
+  a.out_ids = buf<uint32_t>(idx, B_NEW_IDS);
+  a.out_len = buf<uint32_t>(idx, B_NEW_LEN);
+  a.alpha = alpha;
+  a.Rsel = R;
+  a.add = add ? 1 : 0;
+  for (double& p : idx->prof) p = 0;
+  ddg_status st = run_prune(idx, a, np, maxc + (add ? idx->R : 0), &idx->prof[1]);
+  if (st != DDG_OK) return st;
+  HIPCHK(hipMemcpy(out_ids, buf<uint32_t>(idx, B_NEW_IDS), np * R * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out_len, buf<uint32_t>(idx, B_NEW_LEN), np * 4, hipMemcpyDeviceToHost));
+  return DDG_OK;
+}
+
+// Search over device-resident queries/outputs (all pointers device memory).
+ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
+                             const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
+                             const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids, float* out_dists,
+                             uint64_t* dist_cmps, uint32_t* visited) {
+  SearchArgs a{};
+  Launch l{};
+  if (!search_layout(idx, beam_max, nstarts, a, l))
+    return fail(idx, DDG_EINVAL, "beam width too large for the LDS-resident visited filter (max 256)");
+  ENSURE(B_FLAGS, 64);
+  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+  HIPCHK(launch_fill_u32(flags, 0u, 2, idx->stream));
+  a.base = idx->rows;
+  a.graph = idx->graph;
+  a.queries = queries;
+  a.qids = query_ids;
+  a.starts = starts;
+  a.per_query_beam = per_query_beam;
+  a.out_ids = out_ids;
+  a.out_dists = out_dists;
+  a.dist_cmps = dist_cmps;
+  a.visited_count = visited;
+  a.work = flags;
+  a.overflow = flags + 1;
+  a.n = idx->n;
+  a.nq = nq;
+  a.limit = qp->limit ? qp->limit : idx->n;
+  a.cut = qp->cut;
+  a.d = idx->d;
+  a.ad = idx->ad;
+  a.R = idx->R;
+  a.nstarts = nstarts;
+  a.k = qp->k;
+  a.beam = qp->beam;
+  a.degree_limit = qp->degree_limit ? std::min(qp->degree_limit, idx->R) : idx->R;
+  a.build_mode = 0;
+  a.cut_trim = idx->metric == DDG_L2 ? 1 : 0;
+  l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (nq + l.waves_per_block - 1) / l.waves_per_block);
+  if (l.blocks == 0) return DDG_OK;
+  Timer t(idx, &idx->prof[0]);
+  HIPCHK(launch_search(idx->dtype, idx->metric, a, l, idx->stream));
+  t.stop();
+  return DDG_OK;
+}
+
+ddg_status ddg_search(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
+                      const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
+                      const uint32_t* per_query_beam, uint32_t* out_ids, float* out_dists, uint64_t* dist_cmps,
+                      uint32_t* visited) {
+  if (!idx || !qp || (nq && (!queries || !out_ids)) || !starts) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nstarts == 0 || nstarts > 64) return fail(idx, DDG_EINVAL, "nstarts must be in [1, 64]");
+  if (qp->trav != DDG_TRAV_NATIVE) return fail(idx, DDG_EINVAL, "reduced-precision traversal needs ddg_search_lowp");
+  for (uint32_t i = 0; i < nstarts; i++)
+    if (starts[i] >= idx->n) return fail(idx, DDG_ERANGE, "start point out of range");
+  uint32_t beam_max = qp->beam;
+  if (per_query_beam) {
+    beam_max = 0;
+    for (uint64_t i = 0; i < nq; i++) {
+      if (per_query_beam[i] < qp->k || per_query_beam[i] == 0)
+        return fail(idx, DDG_EINVAL, "Error: beam search parameter Q smaller than k");
+      beam_max = std::max(beam_max, per_query_beam[i]);
+    }
+  } else if (qp->k > qp->beam || qp->beam == 0) {
+    return fail(idx, DDG_EINVAL, "Error: beam search parameter Q smaller than k");  // beamSearch.h:327-331
+  }
+  if (qp->k > 0 && qp->cut < 1.0 && idx->metric == DDG_L2)
+    return fail(idx, DDG_EINVAL, "cut below 1.0 is not supported");
+  for (double& p : idx->prof) p = 0;
+  const size_t row_bytes = (size_t)idx->ad * idx->es;
+  ENSURE(B_QUERIES, std::max<uint64_t>(nq, 1) * row_bytes);
+  ENSURE(B_STARTS, 256);
+  ENSURE(B_OUT_IDS, std::max<uint64_t>(nq * qp->k, 1) * 4);
+  ENSURE(B_OUT_D, std::max<uint64_t>(nq * qp->k, 1) * 4);
+  ENSURE(B_CMPS, std::max<uint64_t>(nq, 1) * 8);
+  ENSURE(B_VISN, std::max<uint64_t>(nq, 1) * 4);
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_QUERIES), queries, nq * row_bytes, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(launch_pad_rows(buf<void>(idx, B_QUERIES), nq, idx->d * idx->es, (uint32_t)row_bytes, idx->stream));
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_STARTS), starts, nstarts * 4, hipMemcpyHostToDevice, idx->stream));
+  uint32_t* dqb = nullptr;
+  uint32_t* dqid = nullptr;
+  if (per_query_beam) {
+    ENSURE(B_QBEAM, nq * 4);
+    dqb = buf<uint32_t>(idx, B_QBEAM);
+    HIPCHK(hipMemcpyAsync(dqb, per_query_beam, nq * 4, hipMemcpyHostToDevice, idx->stream));
+  }
+  if (query_ids) {
+    ENSURE(B_QIDS, nq * 4);
+    dqid = buf<uint32_t>(idx, B_QIDS);
+    HIPCHK(hipMemcpyAsync(dqid, query_ids, nq * 4, hipMemcpyHostToDevice, idx->stream));
+  }
+  ddg_status st = ddg_search_device(idx, buf<void>(idx, B_QUERIES), nq, dqid, buf<uint32_t>(idx, B_STARTS), nstarts, qp,
+                                    dqb, beam_max, buf<uint32_t>(idx, B_OUT_IDS), buf<float>(idx, B_OUT_D),
+                                    buf<uint64_t>(idx, B_CMPS), buf<uint32_t>(idx, B_VISN));
+  if (st != DDG_OK) return st;
+  HIPCHK(hipMemcpyAsync(out_ids, buf<void>(idx, B_OUT_IDS), nq * qp->k * 4, hipMemcpyDeviceToHost, idx->stream));
+  if (out_dists)
+    HIPCHK(hipMemcpyAsync(out_dists, buf<void>(idx, B_OUT_D), nq * qp->k * 4, hipMemcpyDeviceToHost, idx->stream));
+  std::vector<uint64_t> cm(nq);
+  std::vector<uint32_t> vs(nq);
+  HIPCHK(hipMemcpyAsync(cm.data(), buf<void>(idx, B_CMPS), nq * 8, hipMemcpyDeviceToHost, idx->stream));
+  HIPCHK(hipMemcpyAsync(vs.data(), buf<void>(idx, B_VISN), nq * 4, hipMemcpyDeviceToHost, idx->stream));
+  HIPCHK(hipStreamSynchronize(idx->stream));
+  double tc = 0, tv = 0;
+  for (uint64_t i = 0; i < nq; i++) { tc += (double)cm[i]; tv += (double)vs[i]; }
+  idx->prof[6] = tc * idx->d * idx->es + tv * 4.0 * (1 + idx->R) + (double)nq * idx->d * idx->es;
+  idx->prof[7] = tc;
+  idx->prof[8] = tv;
+  if (dist_cmps) std::memcpy(dist_cmps, cm.data(), nq * 8);
+  if (visited) std::memcpy(visited, vs.data(), nq * 4);
+  return DDG_OK;
+}
+
+ddg_status ddg_insert_search(ddg_index* idx, const uint32_t* ids, uint64_t m, uint32_t L, uint32_t start_point,
+                             uint32_t vcap, uint32_t* vis_ids, float* vis_dists, uint32_t* vis_len,
+                             uint64_t* dist_cmps) {
+  if (!idx || (m && (!ids || !vis_ids || !vis_dists || !vis_len))) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (L == 0 || vcap == 0) return fail(idx, DDG_EINVAL, "bad L/vcap");
+  if (start_point >= idx->n) return fail(idx, DDG_ERANGE, "start point out of range");
+  for (uint64_t i = 0; i < m; i++)
+    if (ids[i] >= idx->n) return fail(idx, DDG_ERANGE, "point id out of range");
+  if (m == 0) return DDG_OK;
+  for (double& p : idx->prof) p = 0;
+  ENSURE(B_INSERTS, m * 4);
+  HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_INSERTS), ids, m * 4, hipMemcpyHostToDevice));
+  uint32_t cap = vcap;
+  ddg_status st = run_insert_search(idx, buf<uint32_t>(idx, B_INSERTS), m, L, start_point, &cap, nullptr);
+  if (st != DDG_OK) return st;
+  std::vector<uint32_t> vi((size_t)m * cap);
+  std::vector<float> vd((size_t)m * cap);
+  HIPCHK(hipMemcpy(vi.data(), buf<void>(idx, B_VIS_IDS), (size_t)m * cap * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vd.data(), buf<void>(idx, B_VIS_D), (size_t)m * cap * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vis_len, buf<void>(idx, B_VISN), m * 4, hipMemcpyDeviceToHost));
+  if (dist_cmps) HIPCHK(hipMemcpy(dist_cmps, buf<void>(idx, B_CMPS), m * 8, hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < m; i++) {
+    if (vis_len[i] > vcap) return fail(idx, DDG_EINVAL, "visited list longer than vcap");
+    std::memcpy(vis_ids + i * vcap, vi.data() + i * cap, vis_len[i] * 4);
+    std::memcpy(vis_dists + i * vcap, vd.data() + i * cap, vis_len[i] * 4);
+  }
+  return DDG_OK;
+}
+
+ddg_status ddg_ground_truth(ddg_index* idx, const void* queries, uint64_t nq, uint32_t k, uint32_t* ids,
+                            double* dists) {
+  if (!idx || (nq && (!queries || !ids))) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (k == 0 || k > 32 || k > idx->n) return fail(idx, DDG_EINVAL, "k must be in [1, min(32, n)]");
+  if (nq == 0) return DDG_OK;
+  for (double& p : idx->prof) p = 0;
+  const size_t row_bytes = (size_t)idx->ad * idx->es;
+  ENSURE(B_QUERIES, nq * row_bytes);
+  ENSURE(B_OUT_IDS, nq * k * 4);
+  ENSURE(B_GT_D, nq * k * 8);
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_QUERIES), queries, nq * row_bytes, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(launch_pad_rows(buf<void>(idx, B_QUERIES), nq, idx->d * idx->es, (uint32_t)row_bytes, idx->stream));
+  Timer t(idx, &idx->prof[5]);
+  HIPCHK(launch_ground_truth(idx->dtype, idx->metric, idx->rows, idx->n, buf<void>(idx, B_QUERIES), nq, idx->d,
+                             idx->ad, k, buf<uint32_t>(idx, B_OUT_IDS), buf<double>(idx, B_GT_D), nullptr, 0,
+                             idx->stream));
+  t.stop();
+  HIPCHK(hipMemcpy(ids, buf<void>(idx, B_OUT_IDS), nq * k * 4, hipMemcpyDeviceToHost));
+  if (dists) HIPCHK(hipMemcpy(dists, buf<void>(idx, B_GT_D), nq * k * 8, hipMemcpyDeviceToHost));
+  return DDG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,235 @@
+// Device-side building blocks shared by the kernels: wave64 primitives, the
+// visited-filter hash, (distance, id) ordering and the per-dtype distance
+// functions.  Everything here is written for gfx950 (CDNA4, wave64).
+//
+// The whole library is compiled with -ffp-contract=off: the FP32 distance
+// functions below reproduce the reference's summation order exactly and must
+// not be contracted into FMAs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ddg {
+
+constexpr uint32_t kWave = 64;
+constexpr uint32_t kFlag = 0x80000000u;   // frontier entry already visited
+constexpr uint32_t kIdMask = 0x7FFFFFFFu;
+constexpr uint32_t kPruned = 0xFFFFFFFFu; // index.h:102 sentinel (-1)
+
+enum Dtype : int { U8 = 0, I8 = 1, F32 = 2 };
+enum Metric : int { L2 = 0, MIPS = 1 };
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
+
+// number of set bits of m below this lane
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int first_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+
+// Orders this wave's LDS accesses: everything before is visible to every lane
+// of the wave after.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+__device__ __forceinline__ uint32_t bcast0(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+// SplitMix64 finaliser (parlay/utilities.h:145-150).
+__device__ __forceinline__ uint64_t hash64_2(uint64_t x) {
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+  return x ^ (x >> 31);
+}
+
+// beamSearch.h:66: bits = max(10, ceil(log2(beam^2)) - 2), in integers.
+__host__ __device__ __forceinline__ uint32_t filter_bits(uint64_t beam) {
+  uint64_t x = beam * beam;
+  uint32_t c = 0;
+  while (c < 63 && (1ull << c) < x) c++;  // ceil(log2(x)) for x >= 1
+  int b = (int)c - 2;
+  return (uint32_t)(b < 10 ? 10 : b);
+}
+
+// (distance, id) order of beamSearch.h:59-61.
+__device__ __forceinline__ bool less_di(float da, uint32_t ia, float db, uint32_t ib) {
+  return da < db || (da == db && ia < ib);
+}
+
+// #entries of the sorted list (d[], id[] & kIdMask) strictly below (x, xi)
+__device__ __forceinline__ uint32_t lower_bound_di(const float* d, const uint32_t* id, uint32_t n, float x,
+                                                   uint32_t xi) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (less_di(d[mid], id[mid] & kIdMask, x, xi)) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// #entries of the sorted list not above (x, xi)
+__device__ __forceinline__ uint32_t upper_bound_di(const float* d, const uint32_t* id, uint32_t n, float x,
+                                                   uint32_t xi) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (less_di(x, xi, d[mid], id[mid] & kIdMask)) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------------ distances
+// One lane computes one distance between a row in HBM and a reference vector
+// staged in LDS.  Rows are aligned_d elements long, a multiple of 64 bytes,
+// with zero padding (the library zeroes the padding of every device copy).
+//
+// Integer types: squared L2 = |q|^2 + |p|^2 - 2 q.p and the inner product are
+// exact in int32 with the dot4 instructions (|values| < 2^24 at d <= 256), so
+// the float cast gives the reference's bits (euclidian_point.h:44-60,
+// mips_point.h:44-58).
+template <typename T, int METRIC>
+struct Dist;
+
+template <int METRIC>
+struct Dist<uint8_t, METRIC> {
+  struct Q { uint32_t qq; };
+  __device__ static Q prep(const uint8_t* q, uint32_t nchunks) {
+    uint32_t s = 0;
+    const uint4* v = (const uint4*)q;
+    for (uint32_t c = 0; c < nchunks; c++) {
+      uint4 x = v[c];
+      s = __builtin_amdgcn_udot4(x.x, x.x, s, false);
+      s = __builtin_amdgcn_udot4(x.y, x.y, s, false);
+      s = __builtin_amdgcn_udot4(x.z, x.z, s, false);
+      s = __builtin_amdgcn_udot4(x.w, x.w, s, false);
+    }
+    return Q{s};
+  }
+  __device__ static float dist(const uint8_t* row, const uint8_t* q, const Q& st, uint32_t /*d*/,
+                               uint32_t nchunks) {
+    const uint4* r = (const uint4*)row;
+    const uint4* v = (const uint4*)q;
+    uint32_t pp = 0, qp = 0;
+    for (uint32_t c = 0; c < nchunks; c++) {
+      uint4 x = r[c];
+      uint4 y = v[c];
+      qp = __builtin_amdgcn_udot4(x.x, y.x, qp, false);
+      qp = __builtin_amdgcn_udot4(x.y, y.y, qp, false);
+      qp = __builtin_amdgcn_udot4(x.z, y.z, qp, false);
+      qp = __builtin_amdgcn_udot4(x.w, y.w, qp, false);
+      if (METRIC == L2) {
+        pp = __builtin_amdgcn_udot4(x.x, x.x, pp, false);
+        pp = __builtin_amdgcn_udot4(x.y, x.y, pp, false);
+        pp = __builtin_amdgcn_udot4(x.z, x.z, pp, false);
+        pp = __builtin_amdgcn_udot4(x.w, x.w, pp, false);
+      }
+    }
+    if (METRIC == L2) return (float)(int32_t)(st.qq + pp - 2u * qp);
+    return -((float)(int32_t)qp);
+  }
+};
+
+template <int METRIC>
+struct Dist<int8_t, METRIC> {
+  struct Q { int32_t qq; };
+  __device__ static Q prep(const int8_t* q, uint32_t nchunks) {
+    int32_t s = 0;
+    const uint4* v = (const uint4*)q;
+    for (uint32_t c = 0; c < nchunks; c++) {
+      uint4 x = v[c];
+      s = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, s, false);
+      s = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, s, false);
+      s = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, s, false);
+      s = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, s, false);
+    }
+    return Q{s};
+  }
+  __device__ static float dist(const int8_t* row, const int8_t* q, const Q& st, uint32_t /*d*/,
+                               uint32_t nchunks) {
+    const uint4* r = (const uint4*)row;
+    const uint4* v = (const uint4*)q;
+    int32_t pp = 0, qp = 0;
+    for (uint32_t c = 0; c < nchunks; c++) {
+      uint4 x = r[c];
+      uint4 y = v[c];
+      qp = __builtin_amdgcn_sdot4((int)x.x, (int)y.x, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.y, (int)y.y, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.z, (int)y.z, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.w, (int)y.w, qp, false);
+      if (METRIC == L2) {
+        pp = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, pp, false);
+        pp = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, pp, false);
+        pp = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, pp, false);
+        pp = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, pp, false);
+      }
+    }
+    if (METRIC == L2) return (float)(st.qq + pp - 2 * qp);
+    return -((float)qp);
+  }
+};
+
+// FP32: the reference's own order.  L2 follows NSGDist.h:113-131 (per group
+// of four ((e0^2 + e1^2) + e2^2) + e3^2 added to the running sum, then the
+// 0-3 tail elements one by one); MIPS follows mips_point.h:60-66 (sequential
+// sum, negated).
+template <int METRIC>
+struct Dist<float, METRIC> {
+  struct Q { int unused; };
+  __device__ static Q prep(const float*, uint32_t) { return Q{0}; }
+  __device__ static float dist(const float* row, const float* q, const Q&, uint32_t d, uint32_t /*nchunks*/) {
+    const float4* r4 = (const float4*)row;
+    const float4* q4 = (const float4*)q;
+    float acc = 0.f;
+    uint32_t i = 0;
+    if (METRIC == L2) {
+      for (; i + 4 <= d; i += 4) {
+        float4 a = r4[i >> 2];
+        float4 b = q4[i >> 2];
+        float e0 = a.x - b.x, e1 = a.y - b.y, e2 = a.z - b.z, e3 = a.w - b.w;
+        float g = e0 * e0;
+        g = g + e1 * e1;
+        g = g + e2 * e2;
+        g = g + e3 * e3;
+        acc = acc + g;
+      }
+      for (; i < d; i++) {
+        float e = row[i] - q[i];
+        acc = acc + e * e;
+      }
+      return acc;
+    }
+    for (; i + 4 <= d; i += 4) {
+      float4 a = r4[i >> 2];
+      float4 b = q4[i >> 2];
+      acc = acc + b.x * a.x;
+      acc = acc + b.y * a.y;
+      acc = acc + b.z * a.z;
+      acc = acc + b.w * a.w;
+    }
+    for (; i < d; i++) acc = acc + q[i] * row[i];
+    return -acc;
+  }
+};
+
+// Copies one aligned row (bytes a multiple of 16) into LDS with the wave.
+__device__ __forceinline__ void stage_row(void* dst, const void* src, uint32_t bytes) {
+  uint4* o = (uint4*)dst;
+  const uint4* s = (const uint4*)src;
+  for (uint32_t c = lane_id(); c < (bytes >> 4); c += kWave) o[c] = s[c];
+}
+
+}  // namespace ddg

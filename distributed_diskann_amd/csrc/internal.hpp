@@ -1,0 +1,94 @@
+// Internal interfaces between the C-ABI layer (ddg_api.hip) and the kernel
+// launchers.  Not installed; the public surface is include/ddg.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace ddg {
+
+// ------------------------------------------------------------------ search
+struct SearchArgs {
+  const void* base;              // n x ad rows
+  const uint32_t* graph;         // n x (R+1)
+  const void* queries;           // nq x ad rows (query mode)
+  const uint32_t* qids;          // build mode: point id per task; query mode: p.id() (nullable)
+  const uint32_t* starts;
+  const uint32_t* per_query_beam;  // nullable
+  uint32_t* out_ids;             // nq x k (query mode)
+  float* out_dists;              // nullable
+  uint64_t* dist_cmps;           // nullable, nq
+  uint32_t* visited_count;       // nullable, nq
+  uint32_t* vis_ids;             // build mode: nq x vcap, visit order
+  float* vis_dists;
+  uint32_t* overflow;            // build mode: set to 1 when a list exceeds vcap
+  uint32_t* work;                // work-queue counter, zero at launch
+  uint64_t n, nq, limit;
+  double cut;
+  uint32_t d, ad, R, nstarts, k, beam, degree_limit, vcap;
+  uint32_t build_mode, cut_trim;
+  // per-wave LDS layout (bytes)
+  uint32_t lds_per_wave, off_table, off_f0, off_f1, off_keep, off_cand, off_sorted;
+  uint32_t fcap, rcap;
+};
+
+// ------------------------------------------------------------------- prune
+struct PruneArgs {
+  const void* base;
+  uint32_t* graph;             // n x (R1); read for add, written when write_graph
+  const uint32_t* pts;         // task -> point id
+  const uint32_t* a_ids;       // candidate ids
+  const float* a_dists;        // nullable: computed when absent
+  const uint64_t* a_off;       // CSR offsets (np + 1) or null
+  const uint32_t* a_len;       // fixed-stride lists: length per task (when a_off is null)
+  uint32_t* out_ids;           // np x Rsel (when !write_graph)
+  uint32_t* out_len;           // np
+  uint32_t* work;
+  uint32_t* big_tasks;         // tasks whose candidate list exceeds the LDS capacity
+  uint32_t* nbig;
+  uint32_t* status;            // [0]: degree overflow
+  float* gscratch_d;           // global candidate arrays for big tasks (per wave)
+  uint32_t* gscratch_i;
+  uint64_t np;
+  double alpha;
+  uint32_t a_stride, d, ad, R1, Rsel, ccap, add, write_graph, append_if_fits;
+  uint32_t lds_per_wave, off_cand, gcap, big_pass;
+};
+
+struct Launch {
+  uint32_t blocks, waves_per_block, lds_bytes;
+};
+
+// launchers (return hipSuccess or an error code)
+hipError_t launch_search(int dtype, int metric, const SearchArgs& a, const Launch& l, hipStream_t s);
+hipError_t launch_prune(int dtype, int metric, const PruneArgs& a, const Launch& l, hipStream_t s);
+
+// edges.hip
+hipError_t launch_apply_rows(uint32_t* graph, uint32_t R1, const uint32_t* pts, const uint32_t* new_ids,
+                             const uint32_t* new_len, uint32_t Rsel, uint64_t B, uint32_t* status, hipStream_t s);
+hipError_t scan_exclusive_u32(const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total, void* tmp,
+                              size_t tmp_bytes, hipStream_t s);
+size_t scan_tmp_bytes(uint64_t n);
+hipError_t launch_emit_pairs(const uint32_t* pts, const uint32_t* new_ids, const uint32_t* new_len,
+                             const uint64_t* offs, uint32_t Rsel, uint64_t B, uint32_t* keys, uint32_t* vals,
+                             hipStream_t s);
+size_t radix_tmp_bytes(uint64_t n);
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint64_t n,
+                            uint32_t key_bits, void* tmp, size_t tmp_bytes, hipStream_t s, bool* result_in_alt);
+hipError_t launch_group_starts(const uint32_t* keys, uint64_t n, uint32_t* flags, uint64_t* offs,
+                               uint64_t* ngroups_dev, uint32_t* group_keys, uint64_t* group_off, void* tmp,
+                               size_t tmp_bytes, hipStream_t s);
+hipError_t launch_sort_rows(int dtype, int metric, const void* base, uint32_t* graph, uint64_t n, uint32_t R1,
+                            uint32_t d, uint32_t ad, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+hipError_t launch_pad_rows(void* rows, uint64_t n, uint32_t d_bytes, uint32_t row_bytes, hipStream_t s);
+
+// dense.hip
+hipError_t launch_ground_truth(int dtype, int metric, const void* base, uint64_t n, const void* queries,
+                               uint64_t nq, uint32_t d, uint32_t ad, uint32_t k, uint32_t* ids, double* dists,
+                               void* tmp, size_t tmp_bytes, hipStream_t s);
+size_t ground_truth_tmp_bytes(uint64_t n, uint64_t nq, uint32_t k);
+
+}  // namespace ddg

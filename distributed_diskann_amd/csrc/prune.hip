@@ -1,0 +1,192 @@
+// robustPrune (index.h:63-123) — one wavefront per point, persistent launch.
+//
+// Task t prunes point p = pts[t] over candidates A_t (ids, with distances to
+// p or without), plus G[p] when `add` (index.h:70-75).  Candidates are
+// ordered by (distance, id) with a bitonic sort in LDS; the reference's
+// std::sort orders by distance only, so equal-distance candidates may come
+// out in another order there (its tie order is libstdc++'s, not a contract).
+// Then the greedy sweep of index.h:84-106: take the next candidate that is
+// neither p nor pruned, and prune every later candidate p' with
+// alpha * d(p*, p') <= d(p, p') evaluated in double.
+//
+// Modes: forward (write up to Rsel ids per task to out_ids/out_len), and
+// reverse (append_if_fits: if |A| + deg(p) <= Rsel append A to G[p]
+// (graph.h:84-93), else replace G[p] by the pruned list; index.h:309-322).
+//
+// A task whose candidate list exceeds the LDS capacity is queued in
+// big_tasks and re-run by a second launch (big_pass = 1) that keeps its
+// candidates in a per-wave global scratch instead.
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace ddg {
+
+template <typename T, int METRIC, bool BIG>
+__global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using D = Dist<T, METRIC>;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  unsigned char* my = smem + wave * a.lds_per_wave;
+  T* pv = (T*)my;                                   // p's row
+  T* sv = (T*)(my + ((a.ad * sizeof(T) + 15u) & ~15u));  // p*'s row
+  uint32_t* sel = (uint32_t*)(my + 2u * ((a.ad * sizeof(T) + 15u) & ~15u));
+  float* Cd;
+  uint32_t* Ci;
+  uint32_t cap;
+  if (BIG) {
+    const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    Cd = a.gscratch_d + gw * a.gcap;
+    Ci = a.gscratch_i + gw * a.gcap;
+    cap = a.gcap;
+  } else {
+    Cd = (float*)(my + a.off_cand);
+    Ci = (uint32_t*)(Cd + a.ccap);
+    cap = a.ccap;
+  }
+  const uint32_t row_bytes = a.ad * (uint32_t)sizeof(T);
+  const uint32_t nchunks = row_bytes >> 4;
+  const T* base = (const T*)a.base;
+  const uint64_t ntasks = BIG ? (uint64_t)a.nbig[0] : a.np;
+
+  for (;;) {
+    uint32_t tq = 0;
+    if (lane == 0) tq = atomicAdd(a.work, 1u);
+    tq = bcast0(tq);
+    if (tq >= ntasks) break;
+    const uint32_t t = BIG ? a.big_tasks[tq] : tq;
+    const uint32_t p = a.pts[t];
+    uint32_t* grow = a.graph + (uint64_t)p * a.R1;
+    uint64_t a0, alen;
+    if (a.a_off) { a0 = a.a_off[t]; alen = a.a_off[t + 1] - a0; }
+    else { a0 = (uint64_t)t * a.a_stride; alen = a.a_len[t]; }
+    const uint32_t deg = grow[0];
+    const uint32_t gdeg = a.add ? deg : 0u;
+
+    if (a.append_if_fits && alen + deg <= a.Rsel) {
+      // append_neighbors (graph.h:84-93)
+      for (uint32_t j = lane; j < alen; j += kWave) grow[1 + deg + j] = a.a_ids[a0 + j];
+      wave_sync();
+      if (lane == 0) grow[0] = deg + (uint32_t)alen;
+      continue;
+    }
+    const uint64_t M64 = alen + gdeg;
+    if (!BIG && M64 > cap) {
+      if (lane == 0) {
+        uint32_t s = atomicAdd(a.nbig, 1u);
+        a.big_tasks[s] = t;
+      }
+      continue;
+    }
+    if (BIG && M64 > cap) {
+      if (lane == 0) atomicOr(a.status, 2u);
+      continue;
+    }
+    const uint32_t M = (uint32_t)M64;
+    stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
+    wave_sync();
+    const typename D::Q ps = D::prep(pv, nchunks);
+
+    // ---- candidates: A (given or computed distances) then G[p]
+    for (uint32_t j = lane; j < M; j += kWave) {
+      uint32_t id;
+      float dd;
+      if (j < alen) {
+        id = a.a_ids[a0 + j];
+        dd = a.a_dists ? a.a_dists[a0 + j]
+                       : D::dist(base + (uint64_t)id * a.ad, pv, ps, a.d, nchunks);
+      } else {
+        id = grow[1 + (j - (uint32_t)alen)];
+        dd = D::dist(base + (uint64_t)id * a.ad, pv, ps, a.d, nchunks);
+      }
+      Cd[j] = dd;
+      Ci[j] = id;
+    }
+    uint32_t P2 = 1;
+    while (P2 < M) P2 <<= 1;
+    for (uint32_t j = M + lane; j < P2; j += kWave) { Cd[j] = __builtin_inff(); Ci[j] = kPruned; }
+    if (BIG) __threadfence_block();
+    wave_sync();
+
+    // ---- bitonic sort of P2 entries by (distance, id)
+    for (uint32_t kk = 2; kk <= P2; kk <<= 1) {
+      for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+        for (uint32_t i = lane; i < (P2 >> 1); i += kWave) {
+          const uint32_t lo = ((i / jj) * (jj << 1)) + (i % jj);
+          const uint32_t hi = lo + jj;
+          const bool up = (lo & kk) == 0;
+          const float dl = Cd[lo], dh = Cd[hi];
+          const uint32_t il = Ci[lo], ih = Ci[hi];
+          const bool sw = up ? less_di(dh, ih, dl, il) : less_di(dl, il, dh, ih);
+          if (sw) { Cd[lo] = dh; Cd[hi] = dl; Ci[lo] = ih; Ci[hi] = il; }
+        }
+        if (BIG) __threadfence_block();
+        wave_sync();
+      }
+    }
+
+    // ---- greedy selection (index.h:84-106)
+    uint32_t nsel = 0, idx = 0;
+    while (nsel < a.Rsel && idx < M) {
+      int found = -1;
+      for (uint32_t b0 = idx; b0 < M; b0 += kWave) {
+        const uint32_t i = b0 + lane;
+        bool ok = false;
+        if (i < M) {
+          const uint32_t c = Ci[i];
+          ok = c != kPruned && c != p;
+        }
+        const uint64_t m = ballot(ok);
+        if (m) { found = (int)(b0 + (uint32_t)first_lane(m)); break; }
+      }
+      if (found < 0) break;
+      const uint32_t pstar = Ci[found];
+      if (lane == 0) sel[nsel] = pstar;
+      nsel++;
+      idx = (uint32_t)found + 1u;
+      if (nsel >= a.Rsel) break;  // the reference's remaining prunes cannot change the result
+      stage_row(sv, base + (uint64_t)pstar * a.ad, row_bytes);
+      wave_sync();
+      const typename D::Q ss = D::prep(sv, nchunks);
+      for (uint32_t j = idx + lane; j < M; j += kWave) {
+        const uint32_t c = Ci[j];
+        if (c != kPruned) {
+          const float dsp = D::dist(base + (uint64_t)c * a.ad, sv, ss, a.d, nchunks);
+          if (a.alpha * (double)dsp <= (double)Cd[j]) Ci[j] = kPruned;
+        }
+      }
+      if (BIG) __threadfence_block();
+      wave_sync();
+    }
+    wave_sync();
+    if (a.write_graph) {
+      for (uint32_t j = lane; j < nsel; j += kWave) grow[1 + j] = sel[j];
+      wave_sync();
+      if (lane == 0) grow[0] = nsel;
+    } else {
+      for (uint32_t j = lane; j < nsel; j += kWave) a.out_ids[(uint64_t)t * a.Rsel + j] = sel[j];
+      if (lane == 0) a.out_len[t] = nsel;
+    }
+    wave_sync();
+  }
+}
+
+template <typename T, int M>
+static hipError_t launch_t(const PruneArgs& a, const Launch& l, hipStream_t s) {
+  hipFuncSetAttribute((const void*)&k_prune<T, M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes);
+  hipFuncSetAttribute((const void*)&k_prune<T, M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes);
+  if (a.big_pass)
+    hipLaunchKernelGGL((k_prune<T, M, true>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
+  else
+    hipLaunchKernelGGL((k_prune<T, M, false>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s,
+                       a);
+  return hipGetLastError();
+}
+
+hipError_t launch_prune(int dtype, int metric, const PruneArgs& a, const Launch& l, hipStream_t s) {
+  if (dtype == U8) return metric == L2 ? launch_t<uint8_t, L2>(a, l, s) : launch_t<uint8_t, MIPS>(a, l, s);
+  if (dtype == I8) return metric == L2 ? launch_t<int8_t, L2>(a, l, s) : launch_t<int8_t, MIPS>(a, l, s);
+  return metric == L2 ? launch_t<float, L2>(a, l, s) : launch_t<float, MIPS>(a, l, s);
+}
+
+}  // namespace ddg

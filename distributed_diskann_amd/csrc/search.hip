@@ -1,0 +1,308 @@
+// Batched greedy beam search (beamSearch.h:50-184) — one wavefront per
+// query / inserted point, all tasks of a batch in one persistent launch fed
+// by a device work counter.
+//
+// Per-wave LDS:  [query row][visited filter 2^bits u32][frontier A][frontier B]
+//                [kept ids][candidates][sorted candidates]
+//
+// Exactness (ids, distance bits, dist_cmps and |visited| equal to the
+// reference's):
+//  * the visited filter is the reference's lossy table (same size, same
+//    hash64_2 slot, check-then-overwrite in neighbor-list order); lanes that
+//    share a slot within one neighbor list resolve as the sequential loop
+//    would: a lane sees the id written by the closest lower lane with the same
+//    slot, and only the highest such lane writes;
+//  * a neighbor equal to p.id() is skipped without touching the table;
+//  * candidates are kept iff !(dist >= cutoff), cutoff = (float)INT_MAX while
+//    the frontier is not full, else the frontier's last distance;
+//  * the frontier / candidate merge is the multiset std::set_union: a
+//    candidate equal to k frontier entries is emitted only for its copies
+//    beyond k;
+//  * the trim is min(beam, size) then, for k > 0 on a metric space, the
+//    upper_bound against (0, (float)(cut * d_k));
+//  * set_difference(frontier, visited) is tracked with a visited flag on
+//    each frontier entry (bit 31 of the id).  A visited vertex that left the
+//    frontier can never re-enter it (its distance is >= the cutoff, or above
+//    cut * d_k once trimmed, and both only shrink), so the flag is exact for
+//    cut >= 1 (the host rejects cut < 1 with k > 0).
+#include "device.hpp"
+#include "internal.hpp"
+
+namespace ddg {
+
+template <typename T, int METRIC>
+__global__ __launch_bounds__(256) void k_search(SearchArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using D = Dist<T, METRIC>;
+  const uint32_t lane = lane_id();
+  unsigned char* my = smem + (threadIdx.x >> 6) * a.lds_per_wave;
+  T* qv = (T*)my;
+  uint32_t* table = (uint32_t*)(my + a.off_table);
+  float* Fd0 = (float*)(my + a.off_f0);
+  uint32_t* Fi0 = (uint32_t*)(Fd0 + a.fcap);
+  float* Fd1 = (float*)(my + a.off_f1);
+  uint32_t* Fi1 = (uint32_t*)(Fd1 + a.fcap);
+  uint32_t* Kid = (uint32_t*)(my + a.off_keep);
+  float* Cd = (float*)(my + a.off_cand);
+  uint32_t* Ci = (uint32_t*)(Cd + a.rcap);
+  float* Sd = (float*)(my + a.off_sorted);
+  uint32_t* Si = (uint32_t*)(Sd + a.rcap);
+
+  const uint32_t row_bytes = a.ad * (uint32_t)sizeof(T);
+  const uint32_t nchunks = row_bytes >> 4;
+  const uint64_t R1 = (uint64_t)a.R + 1;
+  const T* base = (const T*)a.base;
+
+  for (;;) {
+    uint32_t q = 0;
+    if (lane == 0) q = atomicAdd(a.work, 1u);
+    q = bcast0(q);
+    if (q >= a.nq) break;
+
+    const uint32_t beam = a.per_query_beam ? a.per_query_beam[q] : a.beam;
+    const uint32_t mask = (1u << filter_bits(beam)) - 1u;
+    int64_t self_id;
+    const T* qsrc;
+    if (a.build_mode) {
+      uint32_t pid = a.qids[q];
+      self_id = (int64_t)pid;
+      qsrc = base + (uint64_t)pid * a.ad;
+    } else {
+      self_id = a.qids ? (int64_t)a.qids[q] : (int64_t)q;
+      qsrc = (const T*)a.queries + (uint64_t)q * a.ad;
+    }
+    stage_row(qv, qsrc, row_bytes);
+    for (uint32_t i = lane; i <= mask; i += kWave) table[i] = 0xFFFFFFFFu;
+    wave_sync();
+    const typename D::Q qs = D::prep(qv, nchunks);
+
+    // starting frontier, sorted by (distance, id)
+    const uint32_t ns = a.nstarts;
+    {
+      float sd = 0.f;
+      uint32_t sid = 0;
+      if (lane < ns) {
+        sid = a.starts[lane];
+        sd = D::dist(base + (uint64_t)sid * a.ad, qv, qs, a.d, nchunks);
+        Cd[lane] = sd;
+        Ci[lane] = sid;
+      }
+      wave_sync();
+      if (lane < ns) {
+        uint32_t rank = 0;
+        for (uint32_t t = 0; t < ns; t++) {
+          float dt = Cd[t];
+          uint32_t it = Ci[t];
+          rank += (less_di(dt, it, sd, sid) || (dt == sd && it == sid && t < lane)) ? 1u : 0u;
+        }
+        Fd0[rank] = sd;
+        Fi0[rank] = sid;
+      }
+      wave_sync();
+    }
+    uint32_t fsz = ns;
+    uint32_t cur = 0;
+    uint64_t cmps = ns;
+    uint64_t nvis = 0;
+    int pos = fsz > 0 ? 0 : -1;
+
+    while (pos >= 0 && nvis < a.limit) {
+      float* FD = cur ? Fd1 : Fd0;
+      uint32_t* FI = cur ? Fi1 : Fi0;
+      const float cdist = FD[pos];
+      const uint32_t cid = FI[pos] & kIdMask;
+      wave_sync();
+      if (lane == 0) {
+        FI[pos] = cid | kFlag;
+        if (a.build_mode) {
+          if (nvis < a.vcap) {
+            a.vis_ids[(uint64_t)q * a.vcap + nvis] = cid;
+            a.vis_dists[(uint64_t)q * a.vcap + nvis] = cdist;
+          } else {
+            a.overflow[0] = 1u;
+          }
+        }
+      }
+      nvis++;
+
+      // ---- neighbor filter (beamSearch.h:125-131)
+      const uint32_t* row = a.graph + (uint64_t)cid * R1;
+      const uint32_t deg = row[0];
+      const uint32_t m = deg < a.degree_limit ? deg : a.degree_limit;
+      uint32_t nk = 0;
+      for (uint32_t b0 = 0; b0 < m; b0 += kWave) {
+        const uint32_t j = b0 + lane;
+        const bool valid = j < m;
+        const uint32_t av = valid ? row[1 + j] : 0u;
+        const bool act = valid && ((int64_t)av != self_id);
+        const uint32_t slot = (uint32_t)hash64_2((uint64_t)av) & mask;
+        uint32_t prev = act ? table[slot] : 0xFFFFFFFFu;
+        bool has_next = false;
+        uint64_t it = ballot(act);
+        while (it) {
+          const int jj = first_lane(it);
+          it &= it - 1;
+          const uint32_t sj = readlane_u32(slot, jj);
+          const uint32_t aj = readlane_u32(av, jj);
+          if (sj == slot) {
+            if ((uint32_t)jj < lane) prev = aj;
+            else if ((uint32_t)jj > lane) has_next = true;
+          }
+        }
+        const bool keep = act && (prev != av);
+        wave_sync();
+        if (act && !has_next) table[slot] = av;
+        const uint64_t km = ballot(keep);
+        if (keep) Kid[nk + mbcnt(km)] = av;
+        nk += popc(km);
+        wave_sync();
+      }
+      cmps += nk;
+
+      // ---- distances and cutoff (beamSearch.h:135-145)
+      const float cutoff = (fsz < beam) ? 2147483648.0f : FD[fsz - 1];
+      uint32_t nc = 0;
+      for (uint32_t b0 = 0; b0 < nk; b0 += kWave) {
+        const uint32_t j = b0 + lane;
+        const bool v = j < nk;
+        uint32_t id = 0;
+        float dd = 0.f;
+        if (v) {
+          id = Kid[j];
+          dd = D::dist(base + (uint64_t)id * a.ad, qv, qs, a.d, nchunks);
+        }
+        const bool pass = v && !(dd >= cutoff);
+        const uint64_t pm = ballot(pass);
+        if (pass) {
+          const uint32_t o = nc + mbcnt(pm);
+          Cd[o] = dd;
+          Ci[o] = id;
+        }
+        nc += popc(pm);
+      }
+      wave_sync();
+
+      uint32_t nsz = fsz;
+      if (nc > 0) {
+        // ---- sort candidates by (distance, id) (beamSearch.h:148)
+        for (uint32_t j = lane; j < nc; j += kWave) {
+          const float dj = Cd[j];
+          const uint32_t ij = Ci[j];
+          uint32_t rank = 0;
+          for (uint32_t t = 0; t < nc; t++) {
+            const float dt = Cd[t];
+            const uint32_t itt = Ci[t];
+            rank += (less_di(dt, itt, dj, ij) || (dt == dj && itt == ij && t < j)) ? 1u : 0u;
+          }
+          Sd[rank] = dj;
+          Si[rank] = ij;
+        }
+        wave_sync();
+        // ---- multiset union: keep the copies beyond those already in F
+        uint32_t ne = 0;
+        for (uint32_t b0 = 0; b0 < nc; b0 += kWave) {
+          const uint32_t j = b0 + lane;
+          const bool v = j < nc;
+          bool emit = false;
+          float dj = 0.f;
+          uint32_t ij = 0;
+          if (v) {
+            dj = Sd[j];
+            ij = Si[j];
+            const uint32_t lb = lower_bound_di(FD, FI, fsz, dj, ij);
+            uint32_t eq = 0;
+            while (lb + eq < fsz && FD[lb + eq] == dj && (FI[lb + eq] & kIdMask) == ij) eq++;
+            uint32_t rs = j;
+            while (rs > 0 && Sd[rs - 1] == dj && Si[rs - 1] == ij) rs--;
+            emit = (j - rs) >= eq;
+          }
+          const uint64_t em = ballot(emit);
+          wave_sync();
+          if (emit) {
+            const uint32_t o = ne + mbcnt(em);
+            Cd[o] = dj;
+            Ci[o] = ij;
+          }
+          ne += popc(em);
+          wave_sync();
+        }
+        // ---- merge F and E, keeping the first min(beam, |F|+|E|)
+        const uint32_t total = fsz + ne;
+        nsz = total < beam ? total : beam;
+        float* ND = cur ? Fd0 : Fd1;
+        uint32_t* NI = cur ? Fi0 : Fi1;
+        for (uint32_t i = lane; i < fsz; i += kWave) {
+          const float fd = FD[i];
+          const uint32_t fi = FI[i];
+          const uint32_t p = i + lower_bound_di(Cd, Ci, ne, fd, fi & kIdMask);
+          if (p < nsz) { ND[p] = fd; NI[p] = fi; }
+        }
+        for (uint32_t j = lane; j < ne; j += kWave) {
+          const float ed = Cd[j];
+          const uint32_t ei = Ci[j];
+          const uint32_t p = j + upper_bound_di(FD, FI, fsz, ed, ei);
+          if (p < nsz) { ND[p] = ed; NI[p] = ei; }
+        }
+        wave_sync();
+        cur ^= 1u;
+      }
+      float* FD2 = cur ? Fd1 : Fd0;
+      uint32_t* FI2 = cur ? Fi1 : Fi0;
+      // ---- cut trim (beamSearch.h:162-167)
+      if (a.k > 0 && nsz > a.k && a.cut_trim) {
+        const float thr = (float)(a.cut * (double)FD2[a.k]);
+        uint32_t cut_at = nsz;
+        for (uint32_t b0 = 0; b0 < nsz; b0 += kWave) {
+          const uint32_t i = b0 + lane;
+          bool over = false;
+          if (i < nsz) {
+            const float e = FD2[i];
+            over = (thr < e) || (thr == e && (FI2[i] & kIdMask) > 0u);
+          }
+          const uint64_t om = ballot(over);
+          if (om) { cut_at = b0 + (uint32_t)first_lane(om); break; }
+        }
+        nsz = cut_at;
+      }
+      fsz = nsz;
+      // ---- next unvisited frontier entry (set_difference, beamSearch.h:175-178)
+      pos = -1;
+      for (uint32_t b0 = 0; b0 < fsz; b0 += kWave) {
+        const uint32_t i = b0 + lane;
+        const bool u = i < fsz && !(FI2[i] & kFlag);
+        const uint64_t um = ballot(u);
+        if (um) { pos = (int)(b0 + (uint32_t)first_lane(um)); break; }
+      }
+    }
+
+    if (!a.build_mode) {
+      const float* FD = cur ? Fd1 : Fd0;
+      const uint32_t* FI = cur ? Fi1 : Fi0;
+      for (uint32_t j = lane; j < a.k; j += kWave) {
+        const bool ok = j < fsz;
+        a.out_ids[(uint64_t)q * a.k + j] = ok ? (FI[j] & kIdMask) : 0xFFFFFFFFu;
+        if (a.out_dists) a.out_dists[(uint64_t)q * a.k + j] = ok ? FD[j] : __builtin_inff();
+      }
+    }
+    if (lane == 0) {
+      if (a.dist_cmps) a.dist_cmps[q] = cmps;
+      if (a.visited_count) a.visited_count[q] = (uint32_t)nvis;
+    }
+    wave_sync();
+  }
+}
+
+template <typename T, int M>
+static hipError_t launch_t(const SearchArgs& a, const Launch& l, hipStream_t s) {
+  hipFuncSetAttribute((const void*)&k_search<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes);
+  hipLaunchKernelGGL((k_search<T, M>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_search(int dtype, int metric, const SearchArgs& a, const Launch& l, hipStream_t s) {
+  if (dtype == U8) return metric == L2 ? launch_t<uint8_t, L2>(a, l, s) : launch_t<uint8_t, MIPS>(a, l, s);
+  if (dtype == I8) return metric == L2 ? launch_t<int8_t, L2>(a, l, s) : launch_t<int8_t, MIPS>(a, l, s);
+  return metric == L2 ? launch_t<float, L2>(a, l, s) : launch_t<float, MIPS>(a, l, s);
+}
+
+}  // namespace ddg

@@ -1,0 +1,369 @@
+"""Host-side mirror of the reference's build/search interface over libddg.
+
+``Index`` is the thin owner of one device handle (vectors + graph).  The
+reference-named classes and functions below (``PointRange``, ``Graph``,
+``BuildParams``, ``QueryParams``, ``stats``, ``knn_index``, ``searchAll``,
+``checkRecall``) keep the argument meaning of include/index.h,
+include/utils/beamSearch.h and include/utils/check_nn_recall.h; where the
+reference prints and aborts, these raise ``DdgError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import formats
+from ._lib import (DTYPES, METRICS, TRAV, BuildParamsC, DdgError, QueryParamsC, check, lib)
+
+_U32 = np.uint32
+
+
+def _c(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class Index:
+    """One device copy of a PointRange (aligned rows) and of its Graph."""
+
+    def __init__(self, rows: np.ndarray, d: int, metric: str = "l2", R: int = 32):
+        rows = np.ascontiguousarray(rows)
+        if rows.ndim != 2 or rows.dtype.name not in DTYPES:
+            raise ValueError("rows must be a 2-D uint8/int8/float32 array")
+        self.rows = rows
+        self.d = int(d)
+        self.ad = rows.shape[1]
+        self.n = rows.shape[0]
+        self.R = int(R)
+        self.dtype = rows.dtype.name
+        self.metric = metric
+        h = C.c_void_p()
+        check(lib().ddg_create(DTYPES[self.dtype], METRICS[metric], self.n, self.d, self.ad, _c(rows), self.R,
+                               C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ddg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        check(st, self.h)
+
+    # ------------------------------------------------------------- graph
+    def get_graph(self) -> np.ndarray:
+        out = np.empty((self.n, self.R + 1), _U32)
+        self._chk(lib().ddg_graph_get(self.h, _c(out)))
+        return out
+
+    def set_graph(self, rows: np.ndarray):
+        rows = np.ascontiguousarray(rows, dtype=_U32)
+        if rows.shape != (self.n, self.R + 1):
+            raise ValueError(f"graph rows must be {(self.n, self.R + 1)}")
+        self._chk(lib().ddg_graph_set(self.h, _c(rows)))
+
+    # ------------------------------------------------------------- build
+    def build(self, R=None, L=64, alpha=1.2, two_pass=False, start_point=0, base=2.0, max_fraction=1.0,
+              max_build=0, final_sort=True, bsp=False, visited_counts=False):
+        R = self.R if R is None else R
+        bp = BuildParamsC(R, L, alpha, int(two_pass), start_point, base, max_fraction, max_build, int(final_sort),
+                          int(bsp))
+        vc = np.zeros(self.n, _U32) if visited_counts else None
+        secs = C.c_double(0)
+        self._chk(lib().ddg_build(self.h, C.byref(bp), _c(vc), C.byref(secs)))
+        return (secs.value, vc) if visited_counts else secs.value
+
+    def batch_insert(self, inserts, alpha, L, start_point=0, base=2.0, max_fraction=0.02):
+        ins = np.ascontiguousarray(np.asarray(inserts, _U32))
+        self._chk(lib().ddg_batch_insert(self.h, _c(ins), len(ins), alpha, L, start_point, 0, base, max_fraction))
+
+    def sort_rows(self):
+        self._chk(lib().ddg_sort_rows(self.h))
+
+    def robust_prune(self, pts, cand_lists, cand_dists=None, alpha=1.2, add=True, R=None):
+        R = self.R if R is None else R
+        pts = np.ascontiguousarray(np.asarray(pts, _U32))
+        off = np.zeros(len(pts) + 1, np.uint64)
+        off[1:] = np.cumsum([len(c) for c in cand_lists])
+        ids = np.ascontiguousarray(np.concatenate([np.asarray(c, _U32) for c in cand_lists]) if len(pts) else
+                                   np.zeros(0, _U32))
+        dd = None
+        if cand_dists is not None:
+            dd = np.ascontiguousarray(np.concatenate([np.asarray(c, np.float32) for c in cand_dists]))
+        out = np.zeros((max(len(pts), 1), R), _U32)
+        ln = np.zeros(max(len(pts), 1), _U32)
+        self._chk(lib().ddg_robust_prune(self.h, len(pts), _c(pts), _c(off), _c(ids), _c(dd), alpha, int(add), R,
+                                         _c(out), _c(ln)))
+        return [out[i, :ln[i]].copy() for i in range(len(pts))]
+
+    # ------------------------------------------------------------ search
+    def search(self, Q: np.ndarray, k=10, beam=50, cut=1.35, limit=0, degree_limit=0, starts=(0,),
+               per_query_beam=None, query_ids=None, trav="native", rerank=0):
+        Q = np.ascontiguousarray(Q)
+        if Q.dtype != self.rows.dtype or Q.shape[1] != self.ad:
+            raise ValueError("queries must match the base dtype and aligned dimension")
+        nq = Q.shape[0]
+        qp = QueryParamsC(k, beam, cut, limit, degree_limit, TRAV[trav], rerank)
+        st = np.ascontiguousarray(np.asarray(starts, _U32))
+        pqb = None if per_query_beam is None else np.ascontiguousarray(np.asarray(per_query_beam, _U32))
+        qid = None if query_ids is None else np.ascontiguousarray(np.asarray(query_ids, _U32))
+        ids = np.empty((nq, k), _U32)
+        dists = np.empty((nq, k), np.float32)
+        cmps = np.empty(nq, np.uint64)
+        vis = np.empty(nq, _U32)
+        self._chk(lib().ddg_search(self.h, _c(Q), nq, _c(qid), _c(st), len(st), C.byref(qp), _c(pqb), _c(ids),
+                                   _c(dists), _c(cmps), _c(vis)))
+        return ids, dists, cmps, vis
+
+    def search_device(self, Q, k, beam, out_ids, out_dists=None, cmps=None, visited=None, cut=1.35, limit=0,
+                      degree_limit=0, starts=None, per_query_beam=None, beam_max=None, query_ids=None):
+        """Search over torch CUDA tensors (device-resident inputs and outputs)."""
+        qp = QueryParamsC(k, beam, cut, limit, degree_limit, 0, 0)
+        p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        beam_max = beam if beam_max is None else beam_max
+        nst = 1 if starts is None else starts.numel()
+        self._chk(lib().ddg_search_device(self.h, p(Q), Q.shape[0], p(query_ids), p(starts), nst, C.byref(qp),
+                                          p(per_query_beam), beam_max, p(out_ids), p(out_dists), p(cmps),
+                                          p(visited)))
+
+    def insert_search(self, ids, L, start_point=0, vcap=None):
+        ids = np.ascontiguousarray(np.asarray(ids, _U32))
+        vcap = 4 * L + 64 if vcap is None else vcap
+        m = len(ids)
+        vi = np.zeros((max(m, 1), vcap), _U32)
+        vd = np.zeros((max(m, 1), vcap), np.float32)
+        vl = np.zeros(max(m, 1), _U32)
+        cm = np.zeros(max(m, 1), np.uint64)
+        self._chk(lib().ddg_insert_search(self.h, _c(ids), m, L, start_point, vcap, _c(vi), _c(vd), _c(vl), _c(cm)))
+        return [(vi[i, :vl[i]].copy(), vd[i, :vl[i]].copy()) for i in range(m)], cm[:m].copy()
+
+    def ground_truth(self, Q: np.ndarray, k: int):
+        Q = np.ascontiguousarray(Q)
+        if Q.dtype != self.rows.dtype or Q.shape[1] != self.ad:
+            raise ValueError("queries must match the base dtype and aligned dimension")
+        ids = np.empty((Q.shape[0], k), _U32)
+        d = np.empty((Q.shape[0], k), np.float64)
+        self._chk(lib().ddg_ground_truth(self.h, _c(Q), Q.shape[0], k, _c(ids), _c(d)))
+        return ids, d
+
+    def profile(self):
+        out = np.zeros(9, np.float64)
+        self._chk(lib().ddg_last_profile(self.h, _c(out)))
+        keys = ["search_ms", "prune_ms", "edges_ms", "reverse_ms", "sort_ms", "dense_ms", "search_bytes",
+                "dist_cmps", "visited"]
+        return dict(zip(keys, out.tolist()))
+
+
+# ====================================================================
+# Mirrors of the reference's types and entry points
+# ====================================================================
+@dataclass
+class BuildParams:
+    """types.h:103-142 (Vamana fields)."""
+    R: int
+    L: int
+    alpha: float
+    two_pass: bool = False
+
+
+@dataclass
+class QueryParams:
+    """types.h:145-156."""
+    k: int
+    beamSize: int
+    cut: float
+    limit: int
+    degree_limit: int
+
+
+class PointRange:
+    """point_range.h:53-113: aligned rows plus the Point type's metric."""
+
+    def __init__(self, rows_or_file, dtype: str | None = None, metric: str = "l2", d: int | None = None):
+        if isinstance(rows_or_file, str):
+            self.values, self.dims = formats.read_vectors(rows_or_file, dtype)
+        else:
+            X = np.asarray(rows_or_file)
+            self.dims = X.shape[1] if d is None else d
+            self.values = formats.align_rows(X) if d is None else np.ascontiguousarray(X)
+        self.metric = metric
+        self._index = None
+
+    def size(self):
+        return self.values.shape[0]
+
+    def dimension(self):
+        return self.dims
+
+    def aligned_dimension(self):
+        return self.values.shape[1]
+
+
+class Graph:
+    """graph.h:119-209: n x (R+1) rows [deg, nbrs...]."""
+
+    def __init__(self, maxDeg_or_file, n: int | None = None):
+        if isinstance(maxDeg_or_file, str):
+            self.rows = formats.read_graph(maxDeg_or_file)
+        else:
+            self.rows = np.zeros((n, maxDeg_or_file + 1), _U32)
+
+    def size(self):
+        return self.rows.shape[0]
+
+    def max_degree(self):
+        return self.rows.shape[1] - 1
+
+    def __getitem__(self, i):
+        return self.rows[i, 1:1 + self.rows[i, 0]]
+
+    def save(self, path: str):
+        formats.write_graph(path, self.rows)
+
+
+@dataclass
+class stats:
+    """stats.h:53-88: per-point visited / distance counters."""
+    visited: np.ndarray = field(default_factory=lambda: np.zeros(0, _U32))
+    distances: np.ndarray = field(default_factory=lambda: np.zeros(0, _U32))
+
+    @classmethod
+    def of_size(cls, n):
+        return cls(np.zeros(n, _U32), np.zeros(n, _U32))
+
+    @staticmethod
+    def statistics(s):
+        s = np.sort(np.asarray(s))
+        avg = int(int(s.sum()) / float(len(s))) if len(s) else 0
+        tail = int(s[int(0.99 * float(len(s)))]) if len(s) else 0
+        return avg, tail
+
+    def visited_stats(self):
+        return self.statistics(self.visited)
+
+    def dist_stats(self):
+        return self.statistics(self.distances)
+
+
+def _device_index(Points: PointRange, G: Graph) -> Index:
+    idx = Points._index
+    if idx is None or idx.R != G.max_degree():
+        if idx is not None:
+            idx.close()
+        idx = Index(Points.values, Points.dims, Points.metric, G.max_degree())
+        Points._index = idx
+    return idx
+
+
+class knn_index:
+    """index.h:43-369 (Vamana)."""
+
+    def __init__(self, BP: BuildParams):
+        self.BP = BP
+        self.start_point = 0
+
+    def get_start(self):
+        return self.start_point
+
+    def build_index(self, G: Graph, Points: PointRange, BuildStats: stats | None = None):
+        """index.h:125-137: start point 0, all points, final per-row sort."""
+        self.start_point = 0
+        idx = _device_index(Points, G)
+        idx.set_graph(G.rows)
+        secs, vc = idx.build(R=self.BP.R, L=self.BP.L, alpha=self.BP.alpha, two_pass=self.BP.two_pass,
+                             visited_counts=True)
+        G.rows[...] = idx.get_graph()
+        if BuildStats is not None and len(BuildStats.visited) == len(vc):
+            BuildStats.visited += vc
+        return secs
+
+    def batch_insert(self, inserts, G: Graph, Points: PointRange, BuildStats=None, alpha=1.2, random_order=False,
+                     base=2.0, max_fraction=0.02, print_=True):
+        """index.h:213-341."""
+        if random_order:
+            raise DdgError(1, "random_order is not supported")
+        idx = _device_index(Points, G)
+        idx.set_graph(G.rows)
+        idx.batch_insert(inserts, alpha, self.BP.L, self.start_point, base, max_fraction)
+        G.rows[...] = idx.get_graph()
+
+    def robustPrune(self, p, cand, G: Graph, Points: PointRange, alpha, add=True):
+        """index.h:63-123: ``cand`` is a list of ids or of (id, distance) pairs."""
+        idx = _device_index(Points, G)
+        idx.set_graph(G.rows)
+        cand = list(cand)
+        if cand and isinstance(cand[0], (tuple, list)):
+            ids = [c[0] for c in cand]
+            ds = [c[1] for c in cand]
+            return idx.robust_prune([p], [ids], [ds], alpha, add, self.BP.R)[0]
+        return idx.robust_prune([p], [cand], None, alpha, add, self.BP.R)[0]
+
+
+def searchAll(Query_Points: PointRange, G: Graph, Base_Points: PointRange, QueryStats: stats | None,
+              starting_point, QP: QueryParams, per_query_beam=None):
+    """beamSearch.h:314-346 (query i has p.id() = i); returns nq x k ids."""
+    idx = _device_index(Base_Points, G)
+    idx.set_graph(G.rows)
+    starts = [starting_point] if np.isscalar(starting_point) else list(starting_point)
+    ids, _, cmps, vis = idx.search(Query_Points.values, QP.k, QP.beamSize, QP.cut, QP.limit, QP.degree_limit,
+                                   starts, per_query_beam)
+    if QueryStats is not None and len(QueryStats.visited) == len(vis):
+        QueryStats.visited += vis
+        QueryStats.distances += cmps.astype(_U32)
+    return ids
+
+
+def recall_with_ties(found: np.ndarray, gt_ids: np.ndarray, gt_dists: np.ndarray, k: int) -> float:
+    """check_nn_recall.h:86-109: the first k ground-truth ids plus every later
+    one at the k-th distance count as correct answers."""
+    n = found.shape[0]
+    correct = 0
+    for i in range(n):
+        res = list(gt_ids[i, :k])
+        last = gt_dists[i, k - 1]
+        res += [int(g) for g, dd in zip(gt_ids[i, k:], gt_dists[i, k:]) if dd == last]
+        rep = set(int(x) for x in found[i, :k])
+        correct += sum(1 for r in res if int(r) in rep)
+    return correct / float(k * n)
+
+
+@dataclass
+class nn_result:
+    recall: float
+    QPS: float
+    avg_cmps: float
+    tail_cmps: float
+    avg_visited: float
+    tail_visited: float
+    k: int
+    beamQ: int
+    cut: float
+    num_queries: int
+
+
+def checkRecall(G: Graph, Base_Points: PointRange, Query_Points: PointRange, GT, random: bool, start_point, k,
+                QP: QueryParams):
+    """check_nn_recall.h:34-115: a warm-up search, a timed search, recall with
+    ground-truth ties and QPS = nq / t."""
+    if random:
+        raise DdgError(1, "random start points are not supported; use a fixed start point")
+    gt_ids, gt_d = GT
+    if k > gt_ids.shape[1]:
+        raise DdgError(1, f"{k}@{k} too large for ground truth data of size {gt_ids.shape[1]}")
+    searchAll(Query_Points, G, Base_Points, None, start_point, QP)
+    qs = stats.of_size(Query_Points.size())
+    t = time.perf_counter()
+    ids = searchAll(Query_Points, G, Base_Points, qs, start_point, QP)
+    dt = time.perf_counter() - t
+    rec = recall_with_ties(ids, gt_ids, gt_d, k)
+    ac, tc = qs.dist_stats()
+    av, tv = qs.visited_stats()
+    return nn_result(rec, Query_Points.size() / dt, ac, tc, av, tv, k, QP.beamSize, QP.cut, Query_Points.size())

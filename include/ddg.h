@@ -1,0 +1,167 @@
+/* ddg.h — C-ABI of the MI355X-native Vamana/DiskANN build and search.
+ *
+ * One handle (ddg_index) owns the device copy of one PointRange (vectors) and
+ * one Graph (adjacency rows), plus one HIP stream and the build scratch.  The
+ * memory layouts are the reference's, byte for byte:
+ *   vectors  n rows of aligned_d elements (point_range.h:45-50, 80); the
+ *            library zeroes the padding of its device copy
+ *   graph    n rows of (R+1) u32: [degree, nbr_0 .. nbr_{R-1}] (graph.h:126-128)
+ *
+ * Every call is blocking and stream-synchronous; a handle is not thread-safe
+ * (use one handle per host thread).  The caller owns every host buffer, the
+ * library copies in and out.  No C++ exception crosses this boundary: each
+ * call returns a ddg_status and ddg_last_error() describes the last failure.
+ * The reference has no status codes — it prints and abort()s (graph.h:55-90,
+ * index.h:217-223, beamSearch.h:278-282); include/ddg/knn_index.hpp maps a
+ * non-OK status back to that print + abort().
+ */
+#ifndef DDG_H
+#define DDG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ddg_index ddg_index;
+
+typedef enum {
+  DDG_OK = 0,
+  DDG_EINVAL = 1,  /* bad argument / unsupported parameter combination      */
+  DDG_ERANGE = 2,  /* a point id outside [0, n) (index.h:217-223)           */
+  DDG_EDEGREE = 3, /* an adjacency list would exceed R (graph.h:62-90)      */
+  DDG_EOOM = 4,    /* device allocation failed                             */
+  DDG_EDEVICE = 5, /* HIP runtime error or missing GPU                      */
+  DDG_EIO = 6      /* short or corrupt input                                */
+} ddg_status;
+
+typedef enum { DDG_U8 = 0, DDG_I8 = 1, DDG_F32 = 2 } ddg_dtype;
+
+/* Euclidian_Point (euclidian_point.h:68-107, is_metric) and Mips_Point
+ * (mips_point.h:68-109, not metric: no cut trimming in search). */
+typedef enum { DDG_L2 = 0, DDG_MIPS = 1 } ddg_metric;
+
+/* Vector copy the search traverses with (float32 indices only):
+ * NATIVE = the f32 rows, exact reference arithmetic; BF16 / FP8E4M3 = reduced
+ * copies with per-dimension scales (FP8) and FP32 accumulation, followed by
+ * an FP32 re-rank of the final beam. */
+typedef enum { DDG_TRAV_NATIVE = 0, DDG_TRAV_BF16 = 1, DDG_TRAV_FP8E4M3 = 2 } ddg_trav;
+
+/* BuildParams (types.h:103-142) + the batch_insert schedule knobs
+ * (index.h:213-216) + the BSP keys of para.json (dataset.hpp:54-62). */
+typedef struct {
+  uint32_t R, L;
+  double alpha;
+  int two_pass;         /* index.h:131: a first pass with alpha = 1.0        */
+  uint32_t start_point; /* 0 for knn_index::build_index; start_point_index in BSP */
+  double base;          /* prefix-doubling base, 2 in the reference          */
+  double max_fraction;  /* 1.0 for build_index; xita in BSP                  */
+  uint64_t max_build;   /* BSP "max": insert only the first max points (0 = all) */
+  int final_sort;       /* index.h:133-136 per-row sort (build_index only)   */
+  int bsp;              /* 1: batch sizing over m (bsp_index.hpp:164), 0: over n */
+} ddg_build_params;
+
+/* QueryParams (types.h:145-156) plus the traversal precision. */
+typedef struct {
+  uint32_t k, beam;
+  double cut;
+  uint64_t limit;        /* max visited vertices (0 = n)                     */
+  uint32_t degree_limit; /* 0 = R                                            */
+  ddg_trav trav;
+  uint32_t rerank;       /* beam entries re-ranked in FP32 (0 = all)         */
+} ddg_query_params;
+
+/* Replaces PointRange(char*) + Graph(R, n) (point_range.h:61-97,
+ * graph.h:126-128; diskann.cpp:48-52): copies n rows of aligned_d elements
+ * (aligned_d >= d, any padding content) and allocates a zeroed graph. */
+ddg_status ddg_create(ddg_dtype dtype, ddg_metric metric, uint64_t n, uint32_t d, uint32_t aligned_d,
+                      const void* rows, uint32_t R, ddg_index** out);
+void ddg_destroy(ddg_index* idx);
+const char* ddg_last_error(const ddg_index* idx); /* idx may be NULL */
+
+/* Graph(char*) / Graph::save in memory (graph.h:130-200). */
+ddg_status ddg_graph_set(ddg_index* idx, const uint32_t* rows /* n*(R+1) */);
+ddg_status ddg_graph_get(ddg_index* idx, uint32_t* rows /* n*(R+1) */);
+
+/* knn_index::build_index (index.h:125-137) when bsp == 0, or one rank's
+ * BSP_index::insert at P = 1 (bsp_index.hpp:141-434) when bsp == 1.
+ * visited_counts (nullable, n entries) receives BuildStats.visited
+ * (index.h:280).  build_seconds (nullable) receives the device time of the
+ * build, vectors already resident. */
+ddg_status ddg_build(ddg_index* idx, const ddg_build_params* bp, uint32_t* visited_counts,
+                     double* build_seconds);
+
+/* knn_index::batch_insert (index.h:213-341), random_order = 0. */
+ddg_status ddg_batch_insert(ddg_index* idx, const uint32_t* inserts, uint64_t m, double alpha, uint32_t L,
+                            uint32_t start_point, int random_order, double base, double max_fraction);
+
+/* knn_index::robustPrune (index.h:63-123), batched: task t prunes point
+ * pts[t] over candidates cand_ids[cand_off[t] .. cand_off[t+1]) (with their
+ * distances to pts[t] when cand_dists != NULL, computed otherwise), plus
+ * G[pts[t]] when add != 0.  Writes up to R ids per task into out_ids
+ * (np*R) and the count into out_len.  Candidates are ordered by
+ * (distance, id). */
+ddg_status ddg_robust_prune(ddg_index* idx, uint64_t np, const uint32_t* pts, const uint64_t* cand_off,
+                            const uint32_t* cand_ids, const float* cand_dists, double alpha, int add, uint32_t R,
+                            uint32_t* out_ids, uint32_t* out_len);
+
+/* beam_search / searchAll (beamSearch.h:50-184, 322-346) for nq queries in
+ * one persistent launch.  query_ids gives p.id() for the self-skip of
+ * beamSearch.h:128 (NULL = the query's row index, as searchAll does).
+ * per_query_beam (nullable) gives each query its own beam width.  Outputs
+ * frontier[0:k] ids and distances, dist_cmps and |visited| per query
+ * (nullable). */
+ddg_status ddg_search(ddg_index* idx, const void* queries /* nq*aligned_d */, uint64_t nq,
+                      const uint32_t* query_ids, const uint32_t* starts, uint32_t nstarts,
+                      const ddg_query_params* qp, const uint32_t* per_query_beam, uint32_t* out_ids,
+                      float* out_dists, uint64_t* dist_cmps, uint32_t* visited);
+
+/* ddg_search over device memory: queries, query_ids, starts,
+ * per_query_beam and every output are device pointers (e.g. torch tensors on
+ * the same GPU); beam_max bounds per_query_beam (or equals qp->beam).  No
+ * argument validation beyond the launch shape; the call returns once the
+ * search kernel has finished. */
+ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
+                             const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
+                             const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids, float* out_dists,
+                             uint64_t* dist_cmps, uint32_t* visited);
+
+/* Build-mode traces: beam_search(Points[ids[i]], G, Points, start,
+ * QueryParams(0, L, 0, n, R)) as batch_insert calls it (index.h:277-279).
+ * Writes |visited| and the visited (id, distance) lists, vcap per point, in
+ * visit order, plus dist_cmps. */
+ddg_status ddg_insert_search(ddg_index* idx, const uint32_t* ids, uint64_t m, uint32_t L, uint32_t start_point,
+                             uint32_t vcap, uint32_t* vis_ids, float* vis_dists, uint32_t* vis_len,
+                             uint64_t* dist_cmps);
+
+/* Brute-force top-k of nq queries over the index's vectors: FP64
+ * accumulation for float32 (MFMA tiles), exact integer for uint8/int8, ties
+ * by id.  dists may be NULL. */
+ddg_status ddg_ground_truth(ddg_index* idx, const void* queries /* nq*aligned_d */, uint64_t nq, uint32_t k,
+                            uint32_t* ids, double* dists);
+
+/* index.h:133-136: sort every adjacency list by (distance to its owner, id). */
+ddg_status ddg_sort_rows(ddg_index* idx);
+
+/* Size / layout queries. */
+uint64_t ddg_size(const ddg_index* idx);
+uint32_t ddg_max_degree(const ddg_index* idx);
+uint32_t ddg_dims(const ddg_index* idx);
+uint32_t ddg_aligned_dims(const ddg_index* idx);
+
+/* Kernel timing of the last call (milliseconds of device time per kernel
+ * family, measured with HIP events on the handle's stream), for the
+ * roofline report: [0] search, [1] forward prune, [2] edges (pairs, sort,
+ * groups), [3] reverse append/prune, [4] row sort, [5] dense.  Also the
+ * counters of the last ddg_search: [6] algorithmic bytes
+ * = sum over queries of (dist_cmps * d * s + |visited| * 4 * (1 + R) + d * s),
+ * with s the element size of the traversed copy, [7] total dist_cmps,
+ * [8] total |visited|. */
+ddg_status ddg_last_profile(const ddg_index* idx, double* out9);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DDG_H */

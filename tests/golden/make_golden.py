@@ -1,0 +1,174 @@
+"""Writes the golden fixtures under tests/golden/ (run in a container where
+the reference tree and oracle/_ref/ref_harness exist):
+
+    python tests/golden/make_golden.py
+
+Every expected output here comes from the reference's own code, compiled
+from its headers by oracle/Makefile into oracle/_ref/ref_harness:
+
+  hash.npz        parlay::hash64_2(i), i < 4096
+  dist.npz        Point::distance for random and edge-case rows
+                  (Euclidian_Point / Mips_Point over uint8, int8, float32)
+  groupby.npz     parlay::group_by_key on random (key, value) pairs
+  xs.npz          the reference's random-xs data files (data/random10000),
+                  its ground-truth file, a graph built by the oracle (the
+                  reference's build path, index.h, does not compile here), and
+                  the reference beam_search on that graph for the 1,000
+                  queries at Ls in {10, 20, 50, 100, 200}, k=10, cut=1.35:
+                  frontier[0:k] ids and distances, dist_cmps, |visited|;
+                  plus build-mode traces (k=0, L=50) of 100 points
+  n0.npz          the same search outputs on the seeded 10,000 x 20 N(0,1)
+                  workload (seed 0 base, seed 1 queries)
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+
+from distributed_diskann_amd import datasets, formats  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+REF = "/root/reference"
+HARNESS = O.REF_HARNESS
+LS = [10, 20, 50, 100, 200]
+
+
+def run(*args):
+    subprocess.run([HARNESS, *map(str, args)], check=True, capture_output=True)
+
+
+def read_search(path, nq, with_visited=False):
+    raw = open(path, "rb").read()
+    off = 0
+    out = []
+    for _ in range(nq):
+        fl, vl = np.frombuffer(raw, np.uint32, 2, off); off += 8
+        c = int(np.frombuffer(raw, np.uint64, 1, off)[0]); off += 8
+        fi = np.frombuffer(raw, np.uint32, fl, off).copy(); off += 4 * fl
+        fd = np.frombuffer(raw, np.float32, fl, off).copy(); off += 4 * fl
+        vi = vd = None
+        if with_visited:
+            vi = np.frombuffer(raw, np.uint32, vl, off).copy(); off += 4 * vl
+            vd = np.frombuffer(raw, np.float32, vl, off).copy(); off += 4 * vl
+        out.append((fi, fd, c, int(vl), vi, vd))
+    return out
+
+
+def search_fixture(tmp, dtype, metric, base_file, query_file, graph_file, nq, n, R):
+    res = {}
+    for Ls in LS:
+        out = os.path.join(tmp, f"s{Ls}")
+        run("search", dtype, metric, base_file, query_file, graph_file, 10, Ls, 1.35, n, R, 0, out)
+        rs = read_search(out, nq)
+        ids = np.full((nq, 10), 0xFFFFFFFF, np.uint32)
+        ds = np.full((nq, 10), np.inf, np.float32)
+        for i, (fi, fd, c, vl, _, _) in enumerate(rs):
+            ids[i, :min(10, len(fi))] = fi[:10]
+            ds[i, :min(10, len(fd))] = fd[:10]
+        res[f"ids_{Ls}"] = ids
+        res[f"dists_{Ls}"] = ds
+        res[f"cmps_{Ls}"] = np.array([r[2] for r in rs], np.uint64)
+        res[f"visited_{Ls}"] = np.array([r[3] for r in rs], np.uint32)
+    return res
+
+
+def main():
+    if not (os.path.exists(HARNESS) and os.path.isdir(REF)):
+        sys.exit("needs the reference tree and oracle/_ref/ref_harness (make -C oracle)")
+    tmp = tempfile.mkdtemp()
+
+    # hash64_2
+    run("hash", 4096, os.path.join(tmp, "hash"))
+    h = np.fromfile(os.path.join(tmp, "hash"), np.uint64)
+    np.savez_compressed(os.path.join(HERE, "hash.npz"), ids=np.arange(4096, dtype=np.uint64), hash=h)
+
+    # distances
+    rng = np.random.default_rng(123)
+    dist = {}
+    cases = [("uint8", "l2", 128), ("int8", "l2", 100), ("float32", "l2", 20), ("float32", "l2", 200),
+             ("float32", "mips", 200), ("uint8", "mips", 128), ("int8", "mips", 100), ("float32", "l2", 7),
+             ("float32", "mips", 20)]
+    for dt, met, d in cases:
+        npd = O.NP_DTYPES[dt]
+        if dt == "uint8":
+            A = rng.integers(0, 256, (128, d)).astype(npd); B = rng.integers(0, 256, (128, d)).astype(npd)
+            A[0] = 0; B[0] = 255; A[1] = 255; B[1] = 255
+        elif dt == "int8":
+            A = rng.integers(-128, 128, (128, d)).astype(npd); B = rng.integers(-128, 128, (128, d)).astype(npd)
+            A[0] = -128; B[0] = 127; A[1] = 127; B[1] = -128
+        else:
+            A = (rng.standard_normal((128, d)) * rng.uniform(0.01, 100, (128, 1))).astype(npd)
+            B = (rng.standard_normal((128, d)) * rng.uniform(0.01, 100, (128, 1))).astype(npd)
+            B[0] = A[0]
+        fa, fb, fo = (os.path.join(tmp, x) for x in ("A", "B", "o"))
+        formats.write_vectors(fa, A); formats.write_vectors(fb, B)
+        run("dist", dt, met, fa, fb, fo)
+        key = f"{dt}_{met}_{d}"
+        dist[key + "_a"] = A; dist[key + "_b"] = B
+        dist[key + "_d"] = np.fromfile(fo, np.float32)
+    np.savez_compressed(os.path.join(HERE, "dist.npz"), **dist)
+
+    # group_by_key
+    keys = rng.integers(0, 300, 5000).astype(np.uint32)
+    vals = rng.integers(0, 1 << 20, 5000).astype(np.uint32)
+    pf = os.path.join(tmp, "pairs")
+    np.stack([keys, vals], 1).astype(np.uint32).tofile(pf)
+    run("groupby", pf, os.path.join(tmp, "g"))
+    raw = np.fromfile(os.path.join(tmp, "g"), np.uint32)
+    ng = int(raw[0]) | (int(raw[1]) << 32)
+    pos = 2
+    gk, gl, gv = [], [], []
+    for _ in range(ng):
+        k, ln = int(raw[pos]), int(raw[pos + 1]); pos += 2
+        gk.append(k); gl.append(ln); gv.append(raw[pos:pos + ln]); pos += ln
+    np.savez_compressed(os.path.join(HERE, "groupby.npz"), keys=keys, vals=vals, gkeys=np.array(gk, np.uint32),
+                        glens=np.array(gl, np.uint32), gvals=np.concatenate(gv).astype(np.uint32))
+
+    # random-xs: the reference's data, an oracle-built graph, the reference's search on it
+    xs = os.path.join(REF, "data", "random10000")
+    bf, qf, gf = (os.path.join(xs, x) for x in ("data_10000_20", "queries_1000_20", "gt_10000_1000_20"))
+    X, d = formats.read_vectors(bf, "float32")
+    P = O.Points(X, d)
+    G = O.build(P, 32, 50, 1.2, tie_mode=O.TIE_REFERENCE)
+    gfile = os.path.join(tmp, "xs_graph")
+    formats.write_graph(gfile, G.rows)
+    fx = search_fixture(tmp, "float32", "l2", bf, qf, gfile, 1000, 10000, 32)
+    ids = np.arange(0, 10000, 100, dtype=np.uint32)
+    ids.tofile(os.path.join(tmp, "ins_ids"))
+    run("insert", "float32", "l2", bf, gfile, os.path.join(tmp, "ins_ids"), 50, 0, os.path.join(tmp, "ins"))
+    tr = read_search(os.path.join(tmp, "ins"), len(ids), with_visited=True)
+    fx["ins_ids"] = ids
+    fx["ins_cmps"] = np.array([t[2] for t in tr], np.uint64)
+    fx["ins_vlen"] = np.array([t[3] for t in tr], np.uint32)
+    fx["ins_vis_ids"] = np.concatenate([t[4] for t in tr])
+    fx["ins_vis_dists"] = np.concatenate([t[5] for t in tr])
+    fx["ins_fr_ids"] = np.concatenate([t[0] for t in tr])
+    fx["ins_fr_len"] = np.array([len(t[0]) for t in tr], np.uint32)
+    np.savez_compressed(os.path.join(HERE, "xs.npz"), base_file=np.fromfile(bf, np.uint8),
+                        query_file=np.fromfile(qf, np.uint8), gt_file=np.fromfile(gf, np.uint8),
+                        graph_file=np.fromfile(gfile, np.uint8), **fx)
+
+    # seeded 10,000 x 20 N(0,1) workload
+    Xn, Qn, w = datasets.generate("xs")
+    Pn = O.Points(Xn, w.d)
+    Gn = O.build(Pn, w.R, w.L, w.alpha, tie_mode=O.TIE_REFERENCE)
+    gfn, bfn, qfn = (os.path.join(tmp, x) for x in ("n0_graph", "n0_base", "n0_q"))
+    formats.write_graph(gfn, Gn.rows)
+    formats.write_vectors(bfn, Xn, w.d)
+    formats.write_vectors(qfn, Qn, w.d)
+    fx = search_fixture(tmp, "float32", "l2", bfn, qfn, gfn, Qn.shape[0], Xn.shape[0], w.R)
+    np.savez_compressed(os.path.join(HERE, "n0.npz"), graph_file=np.fromfile(gfn, np.uint8),
+                        base_sha=np.frombuffer(__import__("hashlib").sha256(Xn.tobytes()).digest(), np.uint8), **fx)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+"""GPU index build (prefix-doubling batch insert, robustPrune, reverse edges,
+final row sort) against the oracle.  The GPU orders prune candidates by
+(distance, id); the oracle's TIE_ID mode does the same, so the graphs must be
+identical row for row.  Against the reference's own tie order (std::sort by
+distance only) the contract is recall within 0.005."""
+import numpy as np
+import pytest
+
+from conftest import canon
+from test_gpu_search import _small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_build(Xa, d, metric, R, **kw):
+    from distributed_diskann_amd import Index
+    idx = Index(Xa, d, metric, R)
+    secs = idx.build(R=R, **kw)
+    return idx, idx.get_graph(), secs
+
+
+@pytest.mark.parametrize("dtype,metric,d,alpha", [("float32", "l2", 20, 1.2), ("uint8", "l2", 128, 1.2),
+                                                  ("int8", "l2", 100, 1.2), ("float32", "mips", 64, 1.0),
+                                                  ("uint8", "mips", 32, 1.0)])
+def test_build_identical_to_oracle(oracle, dtype, metric, d, alpha):
+    print(f"[gpu] build {dtype} {metric} d={d}", flush=True)
+    Xa, Qa, P, _ = _small_case(oracle, dtype, metric, 4000, d, seed=11, R=24, L=40)
+    G = oracle.build(P, 24, 40, alpha, tie_mode=oracle.TIE_ID)
+    idx, rows, secs = _gpu_build(Xa, d, metric, 24, L=40, alpha=alpha)
+    assert np.array_equal(canon(rows), canon(G.rows))
+
+
+def test_build_n0_identical_and_recall(oracle, n0):
+    print("[gpu] build n0 (10k x 20, R32 L50)", flush=True)
+    from distributed_diskann_amd.index import recall_with_ties
+    w = n0["w"]
+    P = oracle.Points(n0["X"], w.d)
+    G = oracle.build(P, w.R, w.L, w.alpha, tie_mode=oracle.TIE_ID)
+    idx, rows, secs = _gpu_build(n0["X"], w.d, "l2", w.R, L=w.L, alpha=w.alpha)
+    assert np.array_equal(canon(rows), canon(G.rows))
+    gt_ids, gt_d = idx.ground_truth(n0["Q"], 10)
+    ids, _, _, _ = idx.search(n0["Q"], 10, 50, 1.35)
+    r_gpu = recall_with_ties(ids, gt_ids, gt_d, 10)
+    # the reference's own tie order (graph fixture built like index.h)
+    oi, _, _, _ = oracle.search_all(P, oracle.Graph(n0["G"]), n0["Q"], 10, 50, cut=1.35)
+    r_ref = recall_with_ties(oi, gt_ids, gt_d, 10)
+    assert r_gpu >= r_ref - 0.005
+
+
+def test_build_random_xs_recall(oracle, xs):
+    print("[gpu] build random-xs", flush=True)
+    from distributed_diskann_amd.index import recall_with_ties
+    idx, rows, secs = _gpu_build(xs["X"], xs["d"], "l2", 32, L=50, alpha=1.2)
+    ids, _, _, _ = idx.search(xs["Q"], 10, 50, 1.35)
+    r_gpu = recall_with_ties(ids, xs["gt_ids"], xs["gt_d"], 10)
+    oi, _, _, _ = oracle.search_all(oracle.Points(xs["X"], xs["d"]), oracle.Graph(xs["G"]), xs["Q"], 10, 50,
+                                    cut=1.35)
+    r_ref = recall_with_ties(oi, xs["gt_ids"], xs["gt_d"], 10)
+    assert r_gpu >= r_ref - 0.005
+
+
+@pytest.mark.parametrize("kw", [dict(bsp=True, max_fraction=0.02, start_point=17, final_sort=False),
+                                dict(bsp=True, max_fraction=1.0, max_build=2500, final_sort=False),
+                                dict(two_pass=True), dict(final_sort=False, base=3.0)])
+def test_build_variants(oracle, kw):
+    print(f"[gpu] build variant {kw}", flush=True)
+    Xa, Qa, P, _ = _small_case(oracle, "float32", "l2", 4000, 16, seed=5)
+    G = oracle.build(P, 24, 40, 1.2, tie_mode=oracle.TIE_ID, **kw)
+    idx, rows, secs = _gpu_build(Xa, 16, "l2", 24, L=40, alpha=1.2, **kw)
+    assert np.array_equal(canon(rows), canon(G.rows))
+
+
+def test_batch_insert_subset(oracle):
+    print("[gpu] batch_insert subset", flush=True)
+    from distributed_diskann_amd import Index
+    Xa, Qa, P, _ = _small_case(oracle, "uint8", "l2", 3000, 64, seed=9)
+    ins = np.random.default_rng(2).permutation(3000)[:1800].astype(np.uint32)
+    G = oracle.Graph(np.zeros((3000, 25), np.uint32))
+    oracle.batch_insert(P, G, ins, 1.2, 40, base=2.0, max_fraction=0.02, start=int(ins[0]),
+                        tie_mode=oracle.TIE_ID)
+    idx = Index(Xa, 64, "l2", 24)
+    idx.batch_insert(ins, 1.2, 40, start_point=int(ins[0]), base=2.0, max_fraction=0.02)
+    assert np.array_equal(canon(idx.get_graph()), canon(G.rows))
+
+
+def test_robust_prune_vs_oracle(oracle):
+    print("[gpu] robust_prune", flush=True)
+    from distributed_diskann_amd import Index
+    Xa, Qa, P, G = _small_case(oracle, "int8", "l2", 3000, 100, seed=4)
+    idx = Index(Xa, 100, "l2", 24)
+    idx.set_graph(G.rows)
+    rng = np.random.default_rng(0)
+    pts = rng.integers(0, 3000, 200).astype(np.uint32)
+    cands = [rng.integers(0, 3000, rng.integers(0, 600)).astype(np.uint32) for _ in pts]
+    for add in (True, False):
+        for alpha in (1.0, 1.2):
+            got = idx.robust_prune(pts, cands, None, alpha, add, 24)
+            for p, c, g in zip(pts, cands, got):
+                o = oracle.robust_prune(P, G, int(p), c, None, alpha, add, 24, tie_mode=oracle.TIE_ID)
+                assert list(g) == list(o)
+    # with caller-supplied distances
+    dd = [oracle.distances("int8", "l2", Xa[c], np.repeat(Xa[[p]], len(c), 0), 100) for p, c in zip(pts, cands)]
+    got = idx.robust_prune(pts, cands, dd, 1.2, True, 24)
+    for p, c, d_, g in zip(pts, cands, dd, got):
+        o = oracle.robust_prune(P, G, int(p), c, d_, 1.2, True, 24, tie_mode=oracle.TIE_ID)
+        assert list(g) == list(o)
+
+
+def test_build_errors(oracle):
+    from distributed_diskann_amd import DdgError, Index
+    Xa, Qa, P, G = _small_case(oracle, "float32", "l2", 500, 16, seed=1)
+    idx = Index(Xa, 16, "l2", 24)
+    with pytest.raises(DdgError):
+        idx.batch_insert([0, 1, 10**6], 1.2, 40)  # index.h:217-223
+    with pytest.raises(DdgError):
+        idx.build(R=48, L=40)  # R above the graph's max degree
+    bad = np.zeros((500, 25), np.uint32)
+    bad[3, 0] = 30
+    with pytest.raises(DdgError):
+        idx.set_graph(bad)

@@ -1,0 +1,160 @@
+"""GPU beam search (libddg.so through the C-ABI) against the reference's own
+search outputs (golden vectors) and against the oracle.  Bit-exact: ids,
+distance bits, dist_cmps and |visited|."""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _index(X, d, G, metric="l2"):
+    from distributed_diskann_amd import Index
+    idx = Index(X, d, metric, G.shape[1] - 1)
+    idx.set_graph(G)
+    return idx
+
+
+@pytest.mark.parametrize("Ls", [10, 20, 50, 100, 200])
+def test_search_reference_xs(xs, Ls):
+    print(f"[gpu] search random-xs Ls={Ls}", flush=True)
+    z = xs["z"]
+    idx = _index(xs["X"], xs["d"], xs["G"])
+    ids, dists, cmps, vis = idx.search(xs["Q"], 10, Ls, 1.35)
+    assert np.array_equal(ids, z[f"ids_{Ls}"])
+    assert np.array_equal(dists.view(np.uint32), z[f"dists_{Ls}"].view(np.uint32))
+    assert np.array_equal(cmps, z[f"cmps_{Ls}"])
+    assert np.array_equal(vis, z[f"visited_{Ls}"])
+
+
+@pytest.mark.parametrize("Ls", [10, 50, 200])
+def test_search_reference_n0(n0, Ls):
+    print(f"[gpu] search n0 Ls={Ls}", flush=True)
+    z = n0["z"]
+    idx = _index(n0["X"], n0["d"], n0["G"])
+    ids, dists, cmps, vis = idx.search(n0["Q"], 10, Ls, 1.35)
+    assert np.array_equal(ids, z[f"ids_{Ls}"])
+    assert np.array_equal(dists.view(np.uint32), z[f"dists_{Ls}"].view(np.uint32))
+    assert np.array_equal(cmps, z[f"cmps_{Ls}"])
+    assert np.array_equal(vis, z[f"visited_{Ls}"])
+
+
+def test_search_per_query_beam(oracle, xs):
+    print("[gpu] per-query beam", flush=True)
+    rng = np.random.default_rng(7)
+    beams = np.array([10, 20, 50, 100, 200], np.uint32)[rng.integers(0, 5, xs["Q"].shape[0])]
+    idx = _index(xs["X"], xs["d"], xs["G"])
+    ids, dists, cmps, vis = idx.search(xs["Q"], 10, 200, 1.35, per_query_beam=beams)
+    P = oracle.Points(xs["X"], xs["d"])
+    oi, od, oc, ov = oracle.search_all(P, oracle.Graph(xs["G"]), xs["Q"], 10, 200, cut=1.35, per_query_beam=beams)
+    assert np.array_equal(ids, oi)
+    assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+    assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+
+
+def _small_case(oracle, dtype, metric, n, d, seed, R=24, L=40, alpha=1.2):
+    from distributed_diskann_amd.formats import align_rows
+    rng = np.random.default_rng(seed)
+    if dtype == "uint8":
+        C = rng.uniform(0, 200, (20, d))
+        X = np.clip(np.rint(C[rng.integers(0, 20, n)] + rng.normal(0, 15, (n, d))), 0, 255).astype(np.uint8)
+        Q = np.clip(np.rint(C[rng.integers(0, 20, 300)] + rng.normal(0, 15, (300, d))), 0, 255).astype(np.uint8)
+    elif dtype == "int8":
+        C = rng.normal(0, 30, (20, d))
+        X = np.clip(np.rint(C[rng.integers(0, 20, n)] + rng.normal(0, 12, (n, d))), -128, 127).astype(np.int8)
+        Q = np.clip(np.rint(C[rng.integers(0, 20, 300)] + rng.normal(0, 12, (300, d))), -128, 127).astype(np.int8)
+    else:
+        X = rng.standard_normal((n, d)).astype(np.float32)
+        Q = rng.standard_normal((300, d)).astype(np.float32)
+    Xa, Qa = align_rows(X), align_rows(Q)
+    P = oracle.Points(Xa, d, metric)
+    G = oracle.build(P, R, L, alpha, tie_mode=oracle.TIE_ID)
+    return Xa, Qa, P, G
+
+
+@pytest.mark.parametrize("dtype,metric,d", [("uint8", "l2", 128), ("int8", "l2", 100), ("float32", "mips", 64),
+                                            ("uint8", "mips", 32), ("float32", "l2", 200), ("int8", "mips", 48)])
+def test_search_vs_oracle_dtypes(oracle, dtype, metric, d):
+    print(f"[gpu] search {dtype} {metric} d={d}", flush=True)
+    Xa, Qa, P, G = _small_case(oracle, dtype, metric, 3000, d, seed=d)
+    idx = _index(Xa, d, G.rows, metric)
+    for Ls in (10, 40, 100):
+        ids, dists, cmps, vis = idx.search(Qa, 10, Ls, 1.35)
+        oi, od, oc, ov = oracle.search_all(P, G, Qa, 10, Ls, cut=1.35)
+        assert np.array_equal(ids, oi)
+        assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+        assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+
+
+def test_search_limits_and_starts(oracle, xs):
+    print("[gpu] limits / degree limits / starts", flush=True)
+    idx = _index(xs["X"], xs["d"], xs["G"])
+    P = oracle.Points(xs["X"], xs["d"])
+    G = oracle.Graph(xs["G"])
+    for limit, dl, starts in [(15, 32, (0,)), (10000, 9, (0,)), (10000, 32, (5, 9000, 77)), (40, 16, (3, 4))]:
+        ids, dists, cmps, vis = idx.search(xs["Q"], 10, 50, 1.35, limit=limit, degree_limit=dl, starts=starts)
+        oi, od, oc, ov = oracle.search_all(P, G, xs["Q"], 10, 50, cut=1.35, limit=limit, degree_limit=dl,
+                                           starts=starts)
+        assert np.array_equal(ids, oi) and np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+        assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+
+
+def test_search_duplicate_edges(oracle, xs):
+    # append_neighbors allows duplicate ids in a row; the lossy filter and the
+    # multiset merge must then behave exactly as the reference's
+    print("[gpu] duplicate edges", flush=True)
+    G = xs["G"].copy()
+    rng = np.random.default_rng(1)
+    for i in rng.integers(0, G.shape[0], 3000):
+        deg = G[i, 0]
+        if 2 <= deg < 32:
+            G[i, 1 + deg] = G[i, 1 + rng.integers(0, deg)]
+            G[i, 0] = deg + 1
+    idx = _index(xs["X"], xs["d"], G)
+    P = oracle.Points(xs["X"], xs["d"])
+    for Ls in (10, 50):
+        ids, dists, cmps, vis = idx.search(xs["Q"], 10, Ls, 1.35)
+        oi, od, oc, ov = oracle.search_all(P, oracle.Graph(G), xs["Q"], 10, Ls, cut=1.35)
+        assert np.array_equal(ids, oi) and np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+
+
+def test_insert_traces_reference(xs):
+    print("[gpu] build-mode traces", flush=True)
+    z = xs["z"]
+    idx = _index(xs["X"], xs["d"], xs["G"])
+    lists, cmps = idx.insert_search(z["ins_ids"], 50, 0)
+    assert np.array_equal(cmps, z["ins_cmps"])
+    off = 0
+    for i, (vi, vd) in enumerate(lists):
+        vl = int(z["ins_vlen"][i])
+        assert len(vi) == vl
+        order = np.lexsort((vi, vd))  # the reference keeps visited sorted by (distance, id)
+        assert np.array_equal(vi[order], z["ins_vis_ids"][off:off + vl])
+        assert np.array_equal(vd[order].view(np.uint32), z["ins_vis_dists"][off:off + vl].view(np.uint32))
+        off += vl
+
+
+def test_search_errors(xs):
+    from distributed_diskann_amd import DdgError
+    idx = _index(xs["X"], xs["d"], xs["G"])
+    with pytest.raises(DdgError):
+        idx.search(xs["Q"][:4], 20, 10, 1.35)  # k > beam (beamSearch.h:327-331)
+    with pytest.raises(DdgError):
+        idx.search(xs["Q"][:4], 10, 20, 1.35, starts=(10**6,))
+    with pytest.raises(DdgError):
+        idx.search(xs["Q"][:4], 10, 20, 0.9)
+
+
+def test_ground_truth_vs_oracle(oracle, xs):
+    print("[gpu] ground truth", flush=True)
+    idx = _index(xs["X"], xs["d"], xs["G"])
+    ids, d = idx.ground_truth(xs["Q"][:300], 10)
+    oi, od = oracle.ground_truth(oracle.Points(xs["X"], xs["d"]), xs["Q"][:300], 10)
+    assert np.array_equal(ids, oi) and np.array_equal(d, od)
+    for dtype, metric, dd in [("uint8", "l2", 128), ("int8", "mips", 100)]:
+        Xa, Qa, P, G = _small_case(oracle, dtype, metric, 2000, dd, seed=3)
+        idx2 = _index(Xa, dd, G.rows, metric)
+        ids, d = idx2.ground_truth(Qa, 20)
+        oi, od = oracle.ground_truth(P, Qa, 20)
+        assert np.array_equal(ids, oi) and np.array_equal(d, od)
