@@ -1,0 +1,174 @@
+"""Benchmark: index build seconds and search QPS at 10-recall@10 >= 0.95 on
+one MI355X, for the 100,000 x 20 float32 workload (seed 2 base, seed 3
+queries; R=32, L=64, alpha=1.2; k=10).
+
+A step is one batched search of the 10,000 queries (device-resident) at the
+smallest uniform beam width whose 10-recall@10 (FP64 brute-force ground
+truth, ties counted as in check_nn_recall.h:86-109) is >= 0.95.  The build
+is timed once (device time, vectors resident), and the per-query-beam launch
+of the workload (beams uniform over {10, 20, 50, 100, 200}) is timed too.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "search QPS at 10-recall@10=0.95 + index build sec, 1 MI355X, low-prec vs FP32"
+HBM_PEAK = 8.0e12
+BEAMS = [10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 45, 50, 55, 60, 65,
+         70, 80, 90, 100, 120, 140, 160, 180, 200]  # check_nn_recall.h:173-175, up to 200
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="s")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the multi-GPU path is "
+              "design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from distributed_diskann_amd import Index, datasets
+    from distributed_diskann_amd.index import recall_with_ties
+
+    X, Q, w = datasets.generate(args.workload)
+    nq = Q.shape[0]
+    log(f"[bench] workload {w.name}: n={w.n} d={w.d} {w.dtype} {w.metric} R={w.R} L={w.L} alpha={w.alpha} nq={nq}")
+    idx = Index(X, w.d, w.metric, w.R)
+    t0 = time.perf_counter()
+    build_s = idx.build(R=w.R, L=w.L, alpha=w.alpha)
+    build_wall = time.perf_counter() - t0
+    bprof = idx.profile()
+    G = idx.get_graph()
+    log(f"[bench] build {build_s:.3f} s device ({build_wall:.3f} s wall); avg degree {G[:, 0].mean():.2f}")
+
+    gt_ids, gt_d = idx.ground_truth(Q, 10)
+    gt_ms = idx.profile()["dense_ms"]
+    log(f"[bench] FP64 ground truth {gt_ms:.1f} ms")
+
+    # recall sweep over uniform beam widths
+    chosen, sweep = None, []
+    for Ls in BEAMS:
+        ids, _, cmps, vis = idx.search(Q, 10, Ls, 1.35)
+        r = recall_with_ties(ids, gt_ids, gt_d, 10)
+        sweep.append((Ls, r))
+        if r >= 0.95 and chosen is None:
+            chosen = Ls
+            break
+    if chosen is None:
+        chosen = BEAMS[-1]
+    log(f"[bench] sweep {sweep[-3:]} -> Ls={chosen}")
+
+    dev = torch.device("cuda:0")
+    Qd = torch.from_numpy(Q).to(dev)
+    starts = torch.zeros(1, dtype=torch.int32, device=dev)
+    out_ids = torch.empty((nq, 10), dtype=torch.int32, device=dev)
+    out_d = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+    cm = torch.empty(nq, dtype=torch.int64, device=dev)
+    vs = torch.empty(nq, dtype=torch.int32, device=dev)
+
+    def step(Ls, pqb=None, bmax=None):
+        idx.search_device(Qd, 10, Ls, out_ids, out_d, cm, vs, cut=1.35, starts=starts, per_query_beam=pqb,
+                          beam_max=bmax)
+
+    for _ in range(args.warmup):
+        step(chosen)
+    torch.cuda.synchronize()
+    kern_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(chosen)
+        kern_ms += idx.profile()["search_ms"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1000.0 / args.steps
+    kern_avg_ms = kern_ms / args.steps
+    qps = nq / (ms_per_step / 1000.0)
+    rec = recall_with_ties(out_ids.cpu().numpy().view(np.uint32), gt_ids, gt_d, 10)
+    tot_cmps = float(cm.sum().item())
+    tot_vis = float(vs.sum().item())
+    s = 4 if w.dtype == "float32" else 1
+    avg_deg = float(G[:, 0].mean())
+    # algorithmic bytes of one launch: every distance reads one d-element row,
+    # every expansion one adjacency row (1 + degree words), every query its row
+    alg_bytes = tot_cmps * w.d * s + tot_vis * 4.0 * (1.0 + avg_deg) + nq * w.d * s
+    achieved = alg_bytes / (kern_avg_ms / 1000.0)
+    log(f"[bench] Ls={chosen}: {ms_per_step:.3f} ms/step wall, kernel {kern_avg_ms:.3f} ms, QPS {qps:.0f}, "
+        f"recall {rec:.4f}, {achieved / 1e9:.1f} GB/s algorithmic")
+
+    # the per-query-beam launch of the workload
+    beams = datasets.s_query_beams(nq)
+    pqb = torch.from_numpy(beams.astype(np.int32)).to(dev)
+    step(chosen, pqb, int(beams.max()))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(1, args.steps // 4)):
+        step(chosen, pqb, int(beams.max()))
+    torch.cuda.synchronize()
+    mixed_ms = (time.perf_counter() - t0) * 1000.0 / max(1, args.steps // 4)
+    mixed_rec = recall_with_ties(out_ids.cpu().numpy().view(np.uint32), gt_ids, gt_d, 10)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        P = O.Points(X, w.d, w.metric)
+        t0 = time.perf_counter()
+        Gc = O.build(P, w.R, w.L, w.alpha, tie_mode=O.TIE_REFERENCE, nthreads=threads)
+        cpu_build = time.perf_counter() - t0
+        O.search_all(P, Gc, Q, 10, chosen, cut=1.35, nthreads=threads)  # warm-up (check_nn_recall.h:62-66)
+        t0 = time.perf_counter()
+        ci, _, _, _ = O.search_all(P, Gc, Q, 10, chosen, cut=1.35, nthreads=threads)
+        cpu_search = time.perf_counter() - t0
+        cpu_rec = recall_with_ties(ci, gt_ids, gt_d, 10)
+        cpu = {"value": nq / cpu_search, "unit": "queries/s", "cores": threads, "kind": "port",
+               "sample": f"full workload: build of all {w.n} points and {nq} queries at Ls={chosen} "
+                         f"(CPU recall {cpu_rec:.4f})",
+               "build_seconds": cpu_build}
+        log(f"[bench] CPU ({threads} threads): build {cpu_build:.2f} s, QPS {nq / cpu_search:.0f}")
+
+    out = {
+        "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: {w.n}x{w.d} float32 N(0,1) seed {w.base_seed}, {nq} queries seed {w.query_seed}",
+        "config": {"workload": f"{w.n}x{w.d} f32 L2, R={w.R} L={w.L} alpha={w.alpha}, k=10, Ls={chosen}",
+                   "n": w.n, "d": w.d, "nq": nq, "R": w.R, "L": w.L, "alpha": w.alpha, "k": 10, "Ls": chosen},
+        "recall": rec,
+        "build_seconds": build_s,
+        "build_profile_ms": {k: v for k, v in bprof.items() if k.endswith("_ms")},
+        "per_query_beam": {"qps": nq / (mixed_ms / 1000.0), "recall": mixed_rec, "beams": [10, 20, 50, 100, 200]},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "k_search<float,L2>", "kernel_avg_ms": kern_avg_ms,
+                     "note": "12.8 MB of vectors: the working set is cache-resident"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

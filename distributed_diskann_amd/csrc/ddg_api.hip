@@ -52,6 +52,22 @@ struct ddg_index {
 
 static std::string g_err;
 
+// DDG_TRACE=1 prints each build phase to stderr after its kernels finished
+static bool trace_on() {
+  static int on = -1;
+  if (on < 0) { const char* e = getenv("DDG_TRACE"); on = (e && e[0] == '1') ? 1 : 0; }
+  return on == 1;
+}
+#define TRACE(...)                                                     \
+  do {                                                                 \
+    if (trace_on()) {                                                  \
+      (void)hipStreamSynchronize(idx->stream);                         \
+      fprintf(stderr, "[ddg] " __VA_ARGS__);                           \
+      fputc('\n', stderr);                                             \
+      fflush(stderr);                                                  \
+    }                                                                  \
+  } while (0)
+
 #define HIPCHK(expr)                                                                        \
   do {                                                                                      \
     hipError_t e_ = (expr);                                                                 \
@@ -321,8 +337,10 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
     const uint64_t Bn = hi - lo;
     if (Bn == 0) continue;
     const uint32_t* pts = ins + lo;
+    TRACE("batch [%llu, %llu): search", (unsigned long long)lo, (unsigned long long)hi);
     ddg_status st = run_insert_search(idx, pts, Bn, L, start, &vcap, &total_cmps);
     if (st != DDG_OK) return st;
+    TRACE("  search done (vcap %u)", vcap);
     if (vc_dev)
       hipLaunchKernelGGL(k_acc_visited, dim3((Bn + 255) / 256), dim3(256), 0, idx->stream, vc_dev, pts,
                          buf<uint32_t>(idx, B_VISN), Bn);
@@ -342,6 +360,7 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
     pa.add = 1;
     st = run_prune(idx, pa, Bn, vcap + idx->R, &idx->prof[1]);
     if (st != DDG_OK) return st;
+    TRACE("  forward prune done");
     // graph update + pairs (index.h:287-301)
     Timer te(idx, &idx->prof[2]);
     ENSURE(B_OFFS, (size_t)(Bn + 1) * 8);
@@ -357,6 +376,7 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
     uint64_t P = 0;
     HIPCHK(hipMemcpyAsync(&P, dtotal, 8, hipMemcpyDeviceToHost, idx->stream));
     HIPCHK(hipStreamSynchronize(idx->stream));
+    TRACE("  applied rows, %llu pairs", (unsigned long long)P);
     if (P == 0) { te.stop(); continue; }
     ENSURE(B_KEYS, P * 4);
     ENSURE(B_VALS, P * 4);
@@ -390,6 +410,7 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
     uint64_t ngroups = 0;
     std::memcpy(&ngroups, &hostf[6], 8);
     const uint32_t maxg = hostf[8];
+    TRACE("  sorted pairs: %llu groups, largest %u", (unsigned long long)ngroups, maxg);
     // reverse edges: append or prune per target (index.h:309-322)
     PruneArgs ra{};
     ra.pts = buf<uint32_t>(idx, B_GKEYS);
@@ -402,6 +423,7 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
     ra.append_if_fits = 1;
     st = run_prune(idx, ra, ngroups, maxg + idx->R, &idx->prof[3]);
     if (st != DDG_OK) return st;
+    TRACE("  reverse edges done");
   }
   idx->prof[7] += (double)total_cmps;
   return DDG_OK;
@@ -544,6 +566,7 @@ static ddg_status build_common(ddg_index* idx, const ddg_build_params* bp, uint3
     st = batch_insert_dev(idx, buf<uint32_t>(idx, B_INSERTS), m, bp->alpha, bp->L, bp->R, bp->start_point, bp->base,
                           bp->max_fraction, mbb, vc);
   if (st == DDG_OK && bp->final_sort) {
+    TRACE("final row sort");
     Timer t(idx, &idx->prof[4]);
     HIPCHK(launch_sort_rows(idx->dtype, idx->metric, idx->rows, idx->graph, n, idx->R + 1, idx->d, idx->ad,
                             idx->stream));
@@ -770,6 +793,95 @@ ddg_status ddg_insert_search(ddg_index* idx, const uint32_t* ids, uint64_t m, ui
     std::memcpy(vis_ids + i * vcap, vi.data() + i * cap, vis_len[i] * 4);
     std::memcpy(vis_dists + i * vcap, vd.data() + i * cap, vis_len[i] * 4);
   }
+  return DDG_OK;
+}
+
+// Test hook (not part of include/ddg.h): one prune launch in any of the
+// modes the build uses — fixed-stride candidate lists (a_off == NULL) or CSR,
+// forward (out_ids / out_len) or reverse (write_graph, append_if_fits).
+ddg_status ddg_internal_prune(ddg_index* idx, uint64_t np, const uint32_t* pts, const uint32_t* a_ids,
+                              const float* a_dists, const uint32_t* a_len, uint32_t a_stride,
+                              const uint64_t* a_off, uint64_t total_ids, double alpha, uint32_t Rsel, int add,
+                              int write_graph, int append_if_fits, uint32_t max_cand, uint32_t* out_ids,
+                              uint32_t* out_len) {
+  if (!idx || np == 0) return fail(idx, DDG_EINVAL, "bad arguments");
+  ENSURE(B_QIDS, np * 4);
+  ENSURE(B_PRUNE_IDS, std::max<uint64_t>(total_ids, 1) * 4);
+  ENSURE(B_PRUNE_D, std::max<uint64_t>(total_ids, 1) * 4);
+  ENSURE(B_PRUNE_OFF, (np + 1) * 8);
+  ENSURE(B_VISN, np * 4);
+  ENSURE(B_NEW_IDS, np * Rsel * 4);
+  ENSURE(B_NEW_LEN, np * 4);
+  HIPCHK(hipMemcpy(buf<void>(idx, B_QIDS), pts, np * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(buf<void>(idx, B_PRUNE_IDS), a_ids, total_ids * 4, hipMemcpyHostToDevice));
+  if (a_dists) HIPCHK(hipMemcpy(buf<void>(idx, B_PRUNE_D), a_dists, total_ids * 4, hipMemcpyHostToDevice));
+  if (a_off) HIPCHK(hipMemcpy(buf<void>(idx, B_PRUNE_OFF), a_off, (np + 1) * 8, hipMemcpyHostToDevice));
+  if (a_len) HIPCHK(hipMemcpy(buf<void>(idx, B_VISN), a_len, np * 4, hipMemcpyHostToDevice));
+  PruneArgs a{};
+  a.pts = buf<uint32_t>(idx, B_QIDS);
+  a.a_ids = buf<uint32_t>(idx, B_PRUNE_IDS);
+  a.a_dists = a_dists ? buf<float>(idx, B_PRUNE_D) : nullptr;
+  a.a_off = a_off ? buf<uint64_t>(idx, B_PRUNE_OFF) : nullptr;
+  a.a_len = buf<uint32_t>(idx, B_VISN);
+  a.a_stride = a_stride;
+  a.out_ids = buf<uint32_t>(idx, B_NEW_IDS);
+  a.out_len = buf<uint32_t>(idx, B_NEW_LEN);
+  a.alpha = alpha;
+  a.Rsel = Rsel;
+  a.add = add ? 1 : 0;
+  a.write_graph = write_graph ? 1 : 0;
+  a.append_if_fits = append_if_fits ? 1 : 0;
+  ddg_status st = run_prune(idx, a, np, max_cand, &idx->prof[1]);
+  if (st != DDG_OK) return st;
+  if (!write_graph) {
+    HIPCHK(hipMemcpy(out_ids, buf<void>(idx, B_NEW_IDS), np * Rsel * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(out_len, buf<void>(idx, B_NEW_LEN), np * 4, hipMemcpyDeviceToHost));
+  }
+  return DDG_OK;
+}
+
+// Test hook (not part of include/ddg.h): the reverse-edge grouping passes
+// alone.  Sorts n (key, value) pairs stably by key on the device and returns
+// the sorted pairs, the group keys / offsets and the largest group.
+ddg_status ddg_internal_group_pairs(ddg_index* idx, const uint32_t* keys, const uint32_t* vals, uint64_t n,
+                                    uint32_t* out_keys, uint32_t* out_vals, uint64_t* ngroups,
+                                    uint32_t* group_keys, uint64_t* group_off, uint32_t* maxg) {
+  if (!idx || n == 0) return fail(idx, DDG_EINVAL, "bad arguments");
+  for (uint64_t i = 0; i < n; i++)
+    if (keys[i] >= idx->n) return fail(idx, DDG_ERANGE, "key out of range");
+  ENSURE(B_KEYS, n * 4);
+  ENSURE(B_VALS, n * 4);
+  ENSURE(B_KEYS2, n * 4);
+  ENSURE(B_VALS2, n * 4);
+  ENSURE(B_TMP, std::max(scan_tmp_bytes(n), radix_tmp_bytes(n)) + 64);
+  ENSURE(B_GFLAGS, n * 4);
+  ENSURE(B_GOFFS, n * 8);
+  ENSURE(B_GKEYS, n * 4);
+  ENSURE(B_GOFF, (n + 1) * 8);
+  ENSURE(B_FLAGS, 64);
+  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+  HIPCHK(hipMemcpy(buf<void>(idx, B_KEYS), keys, n * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(buf<void>(idx, B_VALS), vals, n * 4, hipMemcpyHostToDevice));
+  bool in_alt = false;
+  HIPCHK(radix_sort_pairs(buf<uint32_t>(idx, B_KEYS), buf<uint32_t>(idx, B_VALS), buf<uint32_t>(idx, B_KEYS2),
+                          buf<uint32_t>(idx, B_VALS2), n, key_bits_for(idx->n), buf<void>(idx, B_TMP),
+                          idx->s.b[B_TMP].cap, idx->stream, &in_alt));
+  uint32_t* skeys = in_alt ? buf<uint32_t>(idx, B_KEYS2) : buf<uint32_t>(idx, B_KEYS);
+  uint32_t* svals = in_alt ? buf<uint32_t>(idx, B_VALS2) : buf<uint32_t>(idx, B_VALS);
+  uint64_t* dng = (uint64_t*)(flags + 10);
+  HIPCHK(launch_group_starts(skeys, n, buf<uint32_t>(idx, B_GFLAGS), buf<uint64_t>(idx, B_GOFFS), dng,
+                             buf<uint32_t>(idx, B_GKEYS), buf<uint64_t>(idx, B_GOFF), buf<void>(idx, B_TMP),
+                             idx->s.b[B_TMP].cap, idx->stream));
+  HIPCHK(launch_fill_u32(flags + 12, 0u, 1, idx->stream));
+  hipLaunchKernelGGL(k_max_group, dim3((n + 255) / 256), dim3(256), 0, idx->stream, buf<uint64_t>(idx, B_GOFF), dng,
+                     flags + 12);
+  HIPCHK(hipStreamSynchronize(idx->stream));
+  HIPCHK(hipMemcpy(out_keys, skeys, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(out_vals, svals, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(ngroups, dng, 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(group_keys, buf<void>(idx, B_GKEYS), *ngroups * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(group_off, buf<void>(idx, B_GOFF), (*ngroups + 1) * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(maxg, flags + 12, 4, hipMemcpyDeviceToHost));
   return DDG_OK;
 }
 

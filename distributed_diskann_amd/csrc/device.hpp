@@ -49,6 +49,17 @@ __device__ __forceinline__ uint32_t bcast0(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
+// Next task index of a persistent wave.  The wave barrier first reconverges
+// every lane: without it the compiler may thread a preceding `if (lane == 0)`
+// block straight into this one, so that lanes 1..63 re-read a stale index
+// while lane 0 takes the atomic (observed as an endless re-run of one task).
+__device__ __forceinline__ uint32_t fetch_task(uint32_t* counter) {
+  wave_sync();
+  uint32_t v = 0;
+  if (lane_id() == 0) v = atomicAdd(counter, 1u);
+  return bcast0(v);
+}
+
 // SplitMix64 finaliser (parlay/utilities.h:145-150).
 __device__ __forceinline__ uint64_t hash64_2(uint64_t x) {
   x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;

@@ -21,68 +21,19 @@
 
 namespace ddg {
 
+// One robustPrune task (index.h:63-110) with the wave: candidates, sort,
+// greedy selection, write-back.
 template <typename T, int METRIC, bool BIG>
-__global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  using D = Dist<T, METRIC>;
-  const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
-  unsigned char* my = smem + wave * a.lds_per_wave;
-  T* pv = (T*)my;                                   // p's row
-  T* sv = (T*)(my + ((a.ad * sizeof(T) + 15u) & ~15u));  // p*'s row
-  uint32_t* sel = (uint32_t*)(my + 2u * ((a.ad * sizeof(T) + 15u) & ~15u));
-  float* Cd;
-  uint32_t* Ci;
-  uint32_t cap;
-  if (BIG) {
-    const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    Cd = a.gscratch_d + gw * a.gcap;
-    Ci = a.gscratch_i + gw * a.gcap;
-    cap = a.gcap;
-  } else {
-    Cd = (float*)(my + a.off_cand);
-    Ci = (uint32_t*)(Cd + a.ccap);
-    cap = a.ccap;
-  }
-  const uint32_t row_bytes = a.ad * (uint32_t)sizeof(T);
-  const uint32_t nchunks = row_bytes >> 4;
-  const T* base = (const T*)a.base;
-  const uint64_t ntasks = BIG ? (uint64_t)a.nbig[0] : a.np;
-
-  for (;;) {
-    uint32_t tq = 0;
-    if (lane == 0) tq = atomicAdd(a.work, 1u);
-    tq = bcast0(tq);
-    if (tq >= ntasks) break;
-    const uint32_t t = BIG ? a.big_tasks[tq] : tq;
-    const uint32_t p = a.pts[t];
-    uint32_t* grow = a.graph + (uint64_t)p * a.R1;
-    uint64_t a0, alen;
-    if (a.a_off) { a0 = a.a_off[t]; alen = a.a_off[t + 1] - a0; }
-    else { a0 = (uint64_t)t * a.a_stride; alen = a.a_len[t]; }
-    const uint32_t deg = grow[0];
-    const uint32_t gdeg = a.add ? deg : 0u;
-
-    if (a.append_if_fits && alen + deg <= a.Rsel) {
-      // append_neighbors (graph.h:84-93)
-      for (uint32_t j = lane; j < alen; j += kWave) grow[1 + deg + j] = a.a_ids[a0 + j];
-      wave_sync();
-      if (lane == 0) grow[0] = deg + (uint32_t)alen;
-      continue;
-    }
-    const uint64_t M64 = alen + gdeg;
-    if (!BIG && M64 > cap) {
-      if (lane == 0) {
-        uint32_t s = atomicAdd(a.nbig, 1u);
-        a.big_tasks[s] = t;
-      }
-      continue;
-    }
-    if (BIG && M64 > cap) {
-      if (lane == 0) atomicOr(a.status, 2u);
-      continue;
-    }
-    const uint32_t M = (uint32_t)M64;
+__device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint32_t p, uint32_t* grow, uint64_t a0,
+                                           uint64_t alen, uint32_t gdeg, T* pv, T* sv, uint32_t* sel, float* Cd,
+                                           uint32_t* Ci) {
+  {
+    using D = Dist<T, METRIC>;
+    const uint32_t lane = lane_id();
+    const uint32_t row_bytes = a.ad * (uint32_t)sizeof(T);
+    const uint32_t nchunks = row_bytes >> 4;
+    const T* base = (const T*)a.base;
+    const uint32_t M = (uint32_t)(alen + gdeg);
     stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
     wave_sync();
     const typename D::Q ps = D::prep(pv, nchunks);
@@ -166,6 +117,61 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
     } else {
       for (uint32_t j = lane; j < nsel; j += kWave) a.out_ids[(uint64_t)t * a.Rsel + j] = sel[j];
       if (lane == 0) a.out_len[t] = nsel;
+    }
+  }
+}
+
+template <typename T, int METRIC, bool BIG>
+__global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  unsigned char* my = smem + wave * a.lds_per_wave;
+  T* pv = (T*)my;                                   // p's row
+  T* sv = (T*)(my + ((a.ad * sizeof(T) + 15u) & ~15u));  // p*'s row
+  uint32_t* sel = (uint32_t*)(my + 2u * ((a.ad * sizeof(T) + 15u) & ~15u));
+  float* Cd;
+  uint32_t* Ci;
+  uint32_t cap;
+  if (BIG) {
+    const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+    Cd = a.gscratch_d + gw * a.gcap;
+    Ci = a.gscratch_i + gw * a.gcap;
+    cap = a.gcap;
+  } else {
+    Cd = (float*)(my + a.off_cand);
+    Ci = (uint32_t*)(Cd + a.ccap);
+    cap = a.ccap;
+  }
+  const uint64_t ntasks = BIG ? (uint64_t)a.nbig[0] : a.np;
+
+  for (;;) {
+    const uint32_t tq = fetch_task(a.work);
+    if (tq >= ntasks) break;
+    const uint32_t t = BIG ? a.big_tasks[tq] : tq;
+    const uint32_t p = a.pts[t];
+    uint32_t* grow = a.graph + (uint64_t)p * a.R1;
+    uint64_t a0, alen;
+    if (a.a_off) { a0 = a.a_off[t]; alen = a.a_off[t + 1] - a0; }
+    else { a0 = (uint64_t)t * a.a_stride; alen = a.a_len[t]; }
+    const uint32_t deg = grow[0];
+    const uint32_t gdeg = a.add ? deg : 0u;
+
+    const uint64_t M64 = alen + gdeg;
+    if (a.append_if_fits && alen + deg <= a.Rsel) {
+      // append_neighbors (graph.h:84-93)
+      for (uint32_t j = lane; j < alen; j += kWave) grow[1 + deg + j] = a.a_ids[a0 + j];
+      wave_sync();
+      if (lane == 0) grow[0] = deg + (uint32_t)alen;
+    } else if (!BIG && M64 > cap) {
+      if (lane == 0) {
+        uint32_t s = atomicAdd(a.nbig, 1u);
+        a.big_tasks[s] = t;
+      }
+    } else if (BIG && M64 > cap) {
+      if (lane == 0) atomicOr(a.status, 2u);
+    } else {
+      prune_task<T, METRIC, BIG>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci);
     }
     wave_sync();
   }

@@ -54,9 +54,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   const T* base = (const T*)a.base;
 
   for (;;) {
-    uint32_t q = 0;
-    if (lane == 0) q = atomicAdd(a.work, 1u);
-    q = bcast0(q);
+    const uint32_t q = fetch_task(a.work);
     if (q >= a.nq) break;
 
     const uint32_t beam = a.per_query_beam ? a.per_query_beam[q] : a.beam;

@@ -118,3 +118,30 @@ def test_build_errors(oracle):
     bad[3, 0] = 30
     with pytest.raises(DdgError):
         idx.set_graph(bad)
+
+
+@pytest.mark.parametrize("n_pairs,n_keys", [(1, 10), (5000, 300), (70000, 4000), (300000, 1 << 20)])
+def test_reverse_pair_grouping(n_pairs, n_keys):
+    # the stable device radix sort + group boundaries of the reverse-edge step,
+    # against numpy's stable argsort (parlay::group_by_key keeps input order)
+    print(f"[gpu] pair grouping {n_pairs} pairs over {n_keys} keys", flush=True)
+    import ctypes as C
+    from distributed_diskann_amd import Index
+    from distributed_diskann_amd._lib import lib
+    rng = np.random.default_rng(n_pairs)
+    idx = Index(np.zeros((n_keys, 16), np.float32), 16, "l2", 8)
+    keys = rng.integers(0, n_keys, n_pairs).astype(np.uint32)
+    vals = np.arange(n_pairs, dtype=np.uint32)
+    ok = np.empty(n_pairs, np.uint32); ov = np.empty(n_pairs, np.uint32)
+    ng = np.zeros(1, np.uint64); gk = np.empty(n_pairs, np.uint32); go = np.empty(n_pairs + 1, np.uint64)
+    mg = np.zeros(1, np.uint32)
+    f = lib().ddg_internal_group_pairs
+    f.restype = C.c_int
+    p = lambda a: C.c_void_p(a.ctypes.data)
+    assert f(idx.h, p(keys), p(vals), C.c_uint64(n_pairs), p(ok), p(ov), p(ng), p(gk), p(go), p(mg)) == 0
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(ok, keys[order]) and np.array_equal(ov, vals[order])
+    uk, starts, counts = np.unique(keys[order], return_index=True, return_counts=True)
+    g = int(ng[0])
+    assert g == len(uk) and np.array_equal(gk[:g], uk) and np.array_equal(go[:g], starts)
+    assert go[g] == n_pairs and mg[0] == counts.max()
