@@ -126,6 +126,8 @@ class Index:
     def search_device(self, Q, k, beam, out_ids, out_dists=None, cmps=None, visited=None, cut=1.35, limit=0,
                       degree_limit=0, starts=None, per_query_beam=None, beam_max=None, query_ids=None):
         """Search over torch CUDA tensors (device-resident inputs and outputs)."""
+        if starts is None:
+            raise ValueError("search_device needs a device tensor of start points")
         qp = QueryParamsC(k, beam, cut, limit, degree_limit, 0, 0)
         p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
         beam_max = beam if beam_max is None else beam_max
