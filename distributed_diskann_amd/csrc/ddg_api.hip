@@ -123,6 +123,7 @@ static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nsta
   a.off_keep = off; off += rcap * 4;
   a.off_cand = off; off += rcap * 8;
   a.off_sorted = off; off += rcap * 8;
+  a.off_bkt = off; off += 1024;
   a.lds_per_wave = round_up(off, 16);
   a.fcap = fcap;
   a.rcap = rcap;

@@ -30,7 +30,7 @@ struct SearchArgs {
   uint32_t d, ad, R, nstarts, k, beam, degree_limit, vcap;
   uint32_t build_mode, cut_trim;
   // per-wave LDS layout (bytes)
-  uint32_t lds_per_wave, off_table, off_f0, off_f1, off_keep, off_cand, off_sorted;
+  uint32_t lds_per_wave, off_table, off_f0, off_f1, off_keep, off_cand, off_sorted, off_bkt;
   uint32_t fcap, rcap;
 };
 

@@ -3,7 +3,7 @@
 // by a device work counter.
 //
 // Per-wave LDS:  [query row][visited filter 2^bits u32][frontier A][frontier B]
-//                [kept ids][candidates][sorted candidates]
+//                [kept ids][candidates][sorted candidates][slot buckets, 1 KB]
 //
 // Exactness (ids, distance bits, dist_cmps and |visited| equal to the
 // reference's):
@@ -30,6 +30,8 @@
 
 namespace ddg {
 
+constexpr uint32_t kBuckets = 1024;  // slot buckets for the lane-conflict check
+
 template <typename T, int METRIC>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -43,6 +45,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   float* Fd1 = (float*)(my + a.off_f1);
   uint32_t* Fi1 = (uint32_t*)(Fd1 + a.fcap);
   uint32_t* Kid = (uint32_t*)(my + a.off_keep);
+  uint8_t* bkt = (uint8_t*)(my + a.off_bkt);
   float* Cd = (float*)(my + a.off_cand);
   uint32_t* Ci = (uint32_t*)(Cd + a.rcap);
   float* Sd = (float*)(my + a.off_sorted);
@@ -103,12 +106,24 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     uint64_t cmps = ns;
     uint64_t nvis = 0;
     int pos = fsz > 0 ? 0 : -1;
+    // Adjacency row of the vertex predicted to be expanded next, loaded while
+    // the current expansion is merged (degree and the first 64 neighbors).
+    uint32_t pf_id = 0xFFFFFFFFu, pf_deg = 0, pf_nb = 0;
 
     while (pos >= 0 && nvis < a.limit) {
       float* FD = cur ? Fd1 : Fd0;
       uint32_t* FI = cur ? Fi1 : Fi0;
       const float cdist = FD[pos];
       const uint32_t cid = FI[pos] & kIdMask;
+      const uint32_t* row = a.graph + (uint64_t)cid * R1;
+      uint32_t deg, nb0;
+      if (pf_id == cid) {
+        deg = pf_deg;
+        nb0 = pf_nb;
+      } else {  // one round trip: the degree and the neighbor words together
+        nb0 = lane < a.R ? row[1 + lane] : 0u;
+        deg = row[0];
+      }
       wave_sync();
       if (lane == 0) {
         FI[pos] = cid | kFlag;
@@ -123,28 +138,60 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       }
       nvis++;
 
+      // the first unvisited frontier entry after cur: the next vertex to
+      // expand unless a new candidate beats it
+      wave_sync();
+      float u_d = __builtin_inff();
+      uint32_t u_id = 0xFFFFFFFFu;
+      for (uint32_t b0 = 0; b0 < fsz; b0 += kWave) {
+        const uint32_t i = b0 + lane;
+        const bool un = i < fsz && !(FI[i] & kFlag);
+        const uint64_t um = ballot(un);
+        if (um) {
+          const uint32_t ui = b0 + (uint32_t)first_lane(um);
+          u_d = FD[ui];
+          u_id = FI[ui] & kIdMask;
+          break;
+        }
+      }
+
       // ---- neighbor filter (beamSearch.h:125-131)
-      const uint32_t* row = a.graph + (uint64_t)cid * R1;
-      const uint32_t deg = row[0];
       const uint32_t m = deg < a.degree_limit ? deg : a.degree_limit;
       uint32_t nk = 0;
       for (uint32_t b0 = 0; b0 < m; b0 += kWave) {
         const uint32_t j = b0 + lane;
         const bool valid = j < m;
-        const uint32_t av = valid ? row[1 + j] : 0u;
+        const uint32_t av = valid ? (b0 == 0 ? nb0 : row[1 + j]) : 0u;
         const bool act = valid && ((int64_t)av != self_id);
         const uint32_t slot = (uint32_t)hash64_2((uint64_t)av) & mask;
         uint32_t prev = act ? table[slot] : 0xFFFFFFFFu;
         bool has_next = false;
-        uint64_t it = ballot(act);
-        while (it) {
-          const int jj = first_lane(it);
-          it &= it - 1;
-          const uint32_t sj = readlane_u32(slot, jj);
-          const uint32_t aj = readlane_u32(av, jj);
-          if (sj == slot) {
-            if ((uint32_t)jj < lane) prev = aj;
-            else if ((uint32_t)jj > lane) has_next = true;
+        // Lanes that share a slot share a bucket of `bkt`.  Every active lane
+        // writes its lane id into its bucket; a lane that reads back another
+        // id lost a race, and only the losers and their buckets' winners go
+        // through the exact lane-order resolution below (usually nobody).
+        const uint32_t bk = slot & (kBuckets - 1u);
+        if (act) bkt[bk] = (uint8_t)lane;
+        wave_sync();
+        const uint32_t owner = act ? (uint32_t)bkt[bk] : lane;
+        const uint64_t lost = ballot(owner != lane);
+        if (lost) {
+          uint64_t conf = lost, it = lost;
+          while (it) {
+            const int l = first_lane(it);
+            it &= it - 1;
+            conf |= 1ull << readlane_u32(owner, l);
+          }
+          it = conf;
+          while (it) {
+            const int jj = first_lane(it);
+            it &= it - 1;
+            const uint32_t sj = readlane_u32(slot, jj);
+            const uint32_t aj = readlane_u32(av, jj);
+            if (act && sj == slot) {
+              if ((uint32_t)jj < lane) prev = aj;
+              else if ((uint32_t)jj > lane) has_next = true;
+            }
           }
         }
         const bool keep = act && (prev != av);
@@ -175,10 +222,23 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
           const uint32_t o = nc + mbcnt(pm);
           Cd[o] = dd;
           Ci[o] = id;
+          if (less_di(dd, id, u_d, u_id)) { u_d = dd; u_id = id; }
         }
         nc += popc(pm);
       }
       wave_sync();
+      // predicted next vertex: the smallest of u and the new candidates
+      for (int o = 1; o < (int)kWave; o <<= 1) {
+        const float od = __shfl_xor(u_d, o);
+        const uint32_t oi = __shfl_xor(u_id, o);
+        if (less_di(od, oi, u_d, u_id)) { u_d = od; u_id = oi; }
+      }
+      pf_id = bcast0(u_id);
+      if (pf_id != 0xFFFFFFFFu) {
+        const uint32_t* prow = a.graph + (uint64_t)pf_id * R1;
+        pf_nb = lane < a.R ? prow[1 + lane] : 0u;
+        pf_deg = prow[0];
+      }
 
       uint32_t nsz = fsz;
       if (nc > 0) {
