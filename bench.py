@@ -1,14 +1,21 @@
-"""Benchmark: index build seconds and search QPS at 10-recall@10 >= 0.95 on
-one MI355X, for the 100,000 x 20 float32 workload (seed 2 base, seed 3
-queries; R=32, L=64, alpha=1.2; k=10).
+"""Benchmark: search QPS at 10-recall@10 >= 0.95 and index build seconds on
+one MI355X.
 
-A step is one batched search of the 10,000 queries (device-resident) at the
-smallest uniform beam width whose 10-recall@10 (FP64 brute-force ground
-truth, ties counted as in check_nn_recall.h:86-109) is >= 0.95.  The build
-is timed once (device time, vectors resident), and the per-query-beam launch
-of the workload (beams uniform over {10, 20, 50, 100, 200}) is timed too.
+Main line (`value`): the 100,000 x 20 float32 workload (seed 2 base, seed 3
+queries; R=32, L=64, alpha=1.2; k=10).  A step is one batched search of the
+10,000 device-resident queries at the smallest uniform beam width whose
+10-recall@10 (FP64 brute-force ground truth, ties counted as in
+check_nn_recall.h:86-109) is >= 0.95.  The build is timed once (device time,
+vectors resident), and the per-query-beam launch of the workload (beams
+uniform over {10, 20, 50, 100, 200}) is timed too.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+`low_precision`: the maximum-inner-product mixture workload (5,000 centres
+N(0,1), scale 0.3, seed 8; queries from shifted centres, seed 9; d=200,
+R=64, L=128, alpha=1.0), at --lowp-n base points: QPS at recall 0.95 for
+FP32, BF16 and FP8-e4m3 traversal (re-rank of the final 100 in FP32), with
+each mode's distance error against FP64.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--lowp-n N]
 
 Prints one JSON line.
 """
@@ -29,10 +36,123 @@ METRIC = "search QPS at 10-recall@10=0.95 + index build sec, 1 MI355X, low-prec 
 HBM_PEAK = 8.0e12
 BEAMS = [10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 45, 50, 55, 60, 65,
          70, 80, 90, 100, 120, 140, 160, 180, 200]  # check_nn_recall.h:173-175, up to 200
+ESIZE = {"native": None, "bf16": 2, "fp8": 1}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+class DeviceSearch:
+    """Device-resident queries and outputs; one call = one search launch."""
+
+    def __init__(self, torch, idx, Q, k=10):
+        dev = torch.device("cuda:0")
+        self.torch, self.idx, self.k = torch, idx, k
+        self.nq = Q.shape[0]
+        self.Qd = torch.from_numpy(Q).to(dev)
+        self.starts = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.ids = torch.empty((self.nq, k), dtype=torch.int32, device=dev)
+        self.d = torch.empty((self.nq, k), dtype=torch.float32, device=dev)
+        self.cm = torch.empty(self.nq, dtype=torch.int64, device=dev)
+        self.vs = torch.empty(self.nq, dtype=torch.int32, device=dev)
+
+    def run(self, Ls, trav="native", rerank=0, pqb=None, bmax=None):
+        self.idx.search_device(self.Qd, self.k, Ls, self.ids, self.d, self.cm, self.vs, cut=1.35,
+                               starts=self.starts, per_query_beam=pqb, beam_max=bmax, trav=trav, rerank=rerank)
+
+    def timed(self, Ls, steps, warmup, trav="native", rerank=0, pqb=None, bmax=None):
+        """(wall ms per step, kernel ms per step) over `steps` launches."""
+        for _ in range(warmup):
+            self.run(Ls, trav, rerank, pqb, bmax)
+        self.torch.cuda.synchronize()
+        kern = 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            before = self.idx.profile()["search_ms"]
+            self.run(Ls, trav, rerank, pqb, bmax)
+            kern += self.idx.profile()["search_ms"] - before
+        self.torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1000.0 / steps, kern / steps
+
+    def result_ids(self):
+        return self.ids.cpu().numpy().view(np.uint32)
+
+    def counters(self):
+        return float(self.cm.sum().item()), float(self.vs.sum().item())
+
+
+def alg_bytes(cmps, vis, nq, d, s_trav, avg_deg, rerank_rows=0):
+    """Algorithmic bytes of one search launch: every distance reads one
+    d-element row of the traversed copy, every expansion one adjacency row
+    (1 + degree words), every query its FP32/native row, and a re-rank reads
+    rerank_rows full FP32 rows per query."""
+    return cmps * d * s_trav + vis * 4.0 * (1.0 + avg_deg) + nq * d * s_trav + nq * rerank_rows * d * 4.0
+
+
+def sweep(ds, gt_ids, gt_d, trav="native", rerank=0, target=0.95):
+    from distributed_diskann_amd.index import recall_with_ties
+    last = None
+    for Ls in BEAMS:
+        ds.run(Ls, trav, rerank)
+        r = recall_with_ties(ds.result_ids(), gt_ids, gt_d, 10)
+        last = (Ls, r)
+        if r >= target:
+            return Ls, r
+    return last
+
+
+def distance_error(idx, X, Q, d, metric, mode, nq=100, m=1000, seed=17):
+    """Relative error of `mode` traversal distances against FP64, normalised by
+    |q| |x| (scale-free for both metrics): median, p99, max."""
+    rng = np.random.default_rng(seed)
+    qi = rng.integers(0, Q.shape[0], nq)
+    ids = rng.integers(0, X.shape[0], (nq, m)).astype(np.uint32)
+    got = idx.distances(Q[qi], ids, mode).astype(np.float64)
+    err = np.empty((nq, m))
+    for i in range(nq):
+        q = Q[qi[i], :d].astype(np.float64)
+        xs = X[ids[i], :d].astype(np.float64)
+        ex = ((xs - q) ** 2).sum(1) if metric == "l2" else -(xs @ q)
+        err[i] = np.abs(got[i] - ex) / (np.linalg.norm(q) * np.linalg.norm(xs, axis=1) + 1e-30)
+    return {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)), "max": float(err.max())}
+
+
+def low_precision(torch, args):
+    from distributed_diskann_amd import Index, datasets
+    X, Q, w = datasets.generate("mips", n=args.lowp_n, nq=args.lowp_nq)
+    nq = Q.shape[0]
+    log(f"[bench] low-precision workload: n={X.shape[0]} d={w.d} mips R={w.R} L={w.L} alpha={w.alpha} nq={nq}")
+    idx = Index(X, w.d, "mips", w.R)
+    build_s = idx.build(R=w.R, L=w.L, alpha=w.alpha)
+    G = idx.get_graph()
+    avg_deg = float(G[:, 0].mean())
+    del G
+    gi, gd = idx.ground_truth(Q, 10)
+    gt_ms = idx.profile()["dense_ms"]
+    log(f"[bench] lowp build {build_s:.2f} s, avg degree {avg_deg:.2f}, FP64 GT {gt_ms:.0f} ms")
+    ds = DeviceSearch(torch, idx, Q)
+    out = {"workload": f"{X.shape[0]}x{w.d} f32 MIPS mixture, R={w.R} L={w.L} alpha={w.alpha}, k=10, nq={nq}",
+           "build_seconds": build_s, "gt_ms": gt_ms, "modes": {}}
+    for mode in ("native", "bf16", "fp8"):
+        rerank = 0 if mode == "native" else 100
+        prep = idx.prepare_traversal(mode)
+        Ls, rec = sweep(ds, gi, gd, mode, rerank)
+        wall, kern = ds.timed(Ls, max(3, args.steps // 4), 1, mode, rerank)
+        cm, vs = ds.counters()
+        s_trav = 4 if mode == "native" else ESIZE[mode]
+        nr = 0 if mode == "native" else min(100, Ls)
+        B = alg_bytes(cm, vs, nq, w.d, s_trav, avg_deg, nr)
+        err = distance_error(idx, X, Q, w.d, "mips", mode) if mode != "native" else None
+        out["modes"][mode] = {"Ls": Ls, "recall": rec, "qps": nq / (wall / 1000.0), "kernel_ms": kern,
+                              "wall_ms": wall, "alg_GBps": B / (kern / 1000.0) / 1e9,
+                              "hbm_frac": B / (kern / 1000.0) / HBM_PEAK, "prepare_s": prep,
+                              "dist_rel_err_vs_fp64": err}
+        log(f"[bench] lowp {mode}: Ls={Ls} recall {rec:.4f} QPS {nq / (wall / 1000.0):.0f} kernel {kern:.3f} ms "
+            f"{B / (kern / 1000.0) / 1e9:.0f} GB/s err {err}")
+    del ds, idx
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -43,6 +163,9 @@ def main():
     ap.add_argument("--workload", default="s")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--lowp-n", type=int, default=1_000_000, help="base points of the low-precision workload "
+                                                                  "(0 skips it)")
+    ap.add_argument("--lowp-nq", type=int, default=10_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the multi-GPU path is "
@@ -63,74 +186,34 @@ def main():
     build_wall = time.perf_counter() - t0
     bprof = idx.profile()
     G = idx.get_graph()
-    log(f"[bench] build {build_s:.3f} s device ({build_wall:.3f} s wall); avg degree {G[:, 0].mean():.2f}")
+    avg_deg = float(G[:, 0].mean())
+    log(f"[bench] build {build_s:.3f} s device ({build_wall:.3f} s wall); avg degree {avg_deg:.2f}")
 
     gt_ids, gt_d = idx.ground_truth(Q, 10)
     gt_ms = idx.profile()["dense_ms"]
     log(f"[bench] FP64 ground truth {gt_ms:.1f} ms")
 
-    # recall sweep over uniform beam widths
-    chosen, sweep = None, []
-    for Ls in BEAMS:
-        ids, _, cmps, vis = idx.search(Q, 10, Ls, 1.35)
-        r = recall_with_ties(ids, gt_ids, gt_d, 10)
-        sweep.append((Ls, r))
-        if r >= 0.95 and chosen is None:
-            chosen = Ls
-            break
-    if chosen is None:
-        chosen = BEAMS[-1]
-    log(f"[bench] sweep {sweep[-3:]} -> Ls={chosen}")
-
-    dev = torch.device("cuda:0")
-    Qd = torch.from_numpy(Q).to(dev)
-    starts = torch.zeros(1, dtype=torch.int32, device=dev)
-    out_ids = torch.empty((nq, 10), dtype=torch.int32, device=dev)
-    out_d = torch.empty((nq, 10), dtype=torch.float32, device=dev)
-    cm = torch.empty(nq, dtype=torch.int64, device=dev)
-    vs = torch.empty(nq, dtype=torch.int32, device=dev)
-
-    def step(Ls, pqb=None, bmax=None):
-        idx.search_device(Qd, 10, Ls, out_ids, out_d, cm, vs, cut=1.35, starts=starts, per_query_beam=pqb,
-                          beam_max=bmax)
-
-    for _ in range(args.warmup):
-        step(chosen)
-    torch.cuda.synchronize()
-    kern_ms = 0.0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        before = idx.profile()["search_ms"]
-        step(chosen)
-        kern_ms += idx.profile()["search_ms"] - before
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1000.0 / args.steps
-    kern_avg_ms = kern_ms / args.steps
+    ds = DeviceSearch(torch, idx, Q)
+    chosen, rec0 = sweep(ds, gt_ids, gt_d)
+    log(f"[bench] sweep -> Ls={chosen} (recall {rec0:.4f})")
+    ms_per_step, kern_avg_ms = ds.timed(chosen, args.steps, args.warmup)
     qps = nq / (ms_per_step / 1000.0)
-    rec = recall_with_ties(out_ids.cpu().numpy().view(np.uint32), gt_ids, gt_d, 10)
-    tot_cmps = float(cm.sum().item())
-    tot_vis = float(vs.sum().item())
+    rec = recall_with_ties(ds.result_ids(), gt_ids, gt_d, 10)
+    tot_cmps, tot_vis = ds.counters()
     s = 4 if w.dtype == "float32" else 1
-    avg_deg = float(G[:, 0].mean())
-    # algorithmic bytes of one launch: every distance reads one d-element row,
-    # every expansion one adjacency row (1 + degree words), every query its row
-    alg_bytes = tot_cmps * w.d * s + tot_vis * 4.0 * (1.0 + avg_deg) + nq * w.d * s
-    achieved = alg_bytes / (kern_avg_ms / 1000.0)
+    achieved = alg_bytes(tot_cmps, tot_vis, nq, w.d, s, avg_deg) / (kern_avg_ms / 1000.0)
     log(f"[bench] Ls={chosen}: {ms_per_step:.3f} ms/step wall, kernel {kern_avg_ms:.3f} ms, QPS {qps:.0f}, "
         f"recall {rec:.4f}, {achieved / 1e9:.1f} GB/s algorithmic")
 
     # the per-query-beam launch of the workload
     beams = datasets.s_query_beams(nq)
-    pqb = torch.from_numpy(beams.astype(np.int32)).to(dev)
-    step(chosen, pqb, int(beams.max()))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(max(1, args.steps // 4)):
-        step(chosen, pqb, int(beams.max()))
-    torch.cuda.synchronize()
-    mixed_ms = (time.perf_counter() - t0) * 1000.0 / max(1, args.steps // 4)
-    mixed_rec = recall_with_ties(out_ids.cpu().numpy().view(np.uint32), gt_ids, gt_d, 10)
+    pqb = torch.from_numpy(beams.astype(np.int32)).to(torch.device("cuda:0"))
+    mixed_ms, mixed_kern = ds.timed(chosen, max(1, args.steps // 4), 1, pqb=pqb, bmax=int(beams.max()))
+    mixed_rec = recall_with_ties(ds.result_ids(), gt_ids, gt_d, 10)
+    log(f"[bench] per-query beams: {mixed_ms:.3f} ms/step, recall {mixed_rec:.4f}")
+    del ds
+
+    lowp = low_precision(torch, args) if args.lowp_n > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -161,10 +244,12 @@ def main():
         "recall": rec,
         "build_seconds": build_s,
         "build_profile_ms": {k: v for k, v in bprof.items() if k.endswith("_ms")},
-        "per_query_beam": {"qps": nq / (mixed_ms / 1000.0), "recall": mixed_rec, "beams": [10, 20, 50, 100, 200]},
+        "per_query_beam": {"qps": nq / (mixed_ms / 1000.0), "kernel_ms": mixed_kern, "recall": mixed_rec,
+                           "beams": [10, 20, 50, 100, 200]},
+        "low_precision": lowp,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_search<float,L2>", "kernel_avg_ms": kern_avg_ms,
+                     "kernel": "k_search<float,L2,native>", "kernel_avg_ms": kern_avg_ms,
                      "note": "12.8 MB of vectors: the working set is cache-resident"},
         "cpu_baseline": cpu,
     }

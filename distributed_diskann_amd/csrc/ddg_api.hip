@@ -44,6 +44,10 @@ struct ddg_index {
   uint32_t* graph = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // reduced traversal copies of a float32 index, indexed by ddg_trav
+  void* trav_rows[3] = {nullptr, nullptr, nullptr};
+  uint32_t trav_stride[3] = {0, 0, 0};
+  float* trav_scale = nullptr;  // FP8: s_j at [0, d), 1/s_j at [d, 2d)
   uint32_t num_cu = 256;
   Scratch s;
   std::string err;
@@ -109,8 +113,40 @@ struct SearchLayout {
   bool ok;
 };
 
-static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, SearchArgs& a, Launch& l) {
-  const uint32_t qrow = round_up(idx->ad * idx->es, 16);
+// Element size and 16-byte chunks per row of a traversal copy.
+static uint32_t trav_esize(const ddg_index* idx, int trav) {
+  return trav == DDG_TRAV_BF16 ? 2u : trav == DDG_TRAV_FP8E4M3 ? 1u : idx->es;
+}
+
+// Row stride of a reduced copy: one 64-byte line for short rows, whole
+// 128-byte lines otherwise.
+static uint32_t trav_stride_for(const ddg_index* idx, int trav) {
+  const uint32_t bytes = idx->d * trav_esize(idx, trav);
+  return bytes <= 64 ? 64u : round_up(bytes, 128);
+}
+
+// Fills the traversal fields of `a` and the query staging sizes; returns the
+// per-wave LDS bytes of the staged query (plus its FP8 factors).
+static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
+  if (trav == DDG_TRAV_NATIVE) {
+    a.qrow_bytes = round_up(idx->ad * idx->es, 16);
+    a.off_qaux = a.qrow_bytes;
+    return a.qrow_bytes;
+  }
+  const uint32_t esz = trav_esize(idx, trav);
+  a.trav_chunks = (idx->d * esz + 15) / 16;
+  a.trav_stride = trav_stride_for(idx, trav);
+  a.trav_rows = idx->trav_rows[trav];
+  a.trav_scale = idx->trav_scale;
+  const uint32_t qn = std::max<uint32_t>(idx->ad, a.trav_chunks * (16 / esz));
+  a.qrow_bytes = round_up(qn * 4, 16);
+  a.off_qaux = a.qrow_bytes;
+  return trav == DDG_TRAV_FP8E4M3 ? 2 * a.qrow_bytes : a.qrow_bytes;
+}
+
+static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, int trav, SearchArgs& a,
+                          Launch& l) {
+  const uint32_t qrow = trav_layout(idx, trav, a);
   const uint32_t bits = filter_bits(beam_max);
   const uint32_t table = 4u << bits;
   const uint32_t fcap = round_up(std::max(beam_max, nstarts), 16);
@@ -191,7 +227,8 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
     HIPCHK(launch_fill_u32(flags, 0u, 16, idx->stream));
     SearchArgs a{};
     Launch l{};
-    if (!search_layout(idx, L, 1, a, l)) return fail(idx, DDG_EINVAL, "build beam L too large for LDS");
+    if (!search_layout(idx, L, 1, DDG_TRAV_NATIVE, a, l))
+      return fail(idx, DDG_EINVAL, "build beam L too large for LDS");
     a.base = idx->rows;
     a.graph = idx->graph;
     a.qids = pts;
@@ -218,7 +255,7 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
     a.cut_trim = 0;
     l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (B + l.waves_per_block - 1) / l.waves_per_block);
     Timer t(idx, &idx->prof[0]);
-    HIPCHK(launch_search(idx->dtype, idx->metric, a, l, idx->stream));
+    HIPCHK(launch_search(idx->dtype, idx->metric, DDG_TRAV_NATIVE, a, l, idx->stream));
     t.stop();
     uint32_t ov = 0;
     HIPCHK(hipMemcpy(&ov, flags + 1, 4, hipMemcpyDeviceToHost));
@@ -430,6 +467,68 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
   return DDG_OK;
 }
 
+// Builds the reduced traversal copy `trav` of a float32 index once per handle.
+static ddg_status prepare_trav(ddg_index* idx, int trav, double* seconds) {
+  if (seconds) *seconds = 0.0;
+  if (trav == DDG_TRAV_NATIVE) return DDG_OK;
+  if (trav != DDG_TRAV_BF16 && trav != DDG_TRAV_FP8E4M3) return fail(idx, DDG_EINVAL, "unknown traversal mode");
+  if (idx->dtype != DDG_F32) return fail(idx, DDG_EINVAL, "reduced-precision traversal needs a float32 index");
+  if (idx->trav_rows[trav]) return DDG_OK;
+  const uint32_t stride = trav_stride_for(idx, trav);
+  void* rows = nullptr;
+  hipError_t e = hipMalloc(&rows, idx->n * (size_t)stride);
+  if (e != hipSuccess)
+    return fail(idx, e == hipErrorOutOfMemory ? DDG_EOOM : DDG_EDEVICE,
+                std::string("traversal copy allocation failed: ") + hipGetErrorString(e));
+  double ms = 0.0;
+  if (trav == DDG_TRAV_BF16) {
+    Timer t(idx, &ms);
+    e = launch_make_bf16((const float*)idx->rows, idx->n, idx->ad, idx->d, rows, stride, idx->stream);
+    t.stop();
+  } else {
+    if (!idx->trav_scale) {
+      e = hipMalloc((void**)&idx->trav_scale, 2 * (size_t)idx->d * 4);
+      if (e != hipSuccess) {
+        (void)hipFree(rows);
+        idx->trav_scale = nullptr;
+        return fail(idx, DDG_EOOM, "scale allocation failed");
+      }
+    }
+    // per-dimension max |x|, scales on the host in IEEE float: s = m / 240, 1/s = 240 / m
+    std::vector<uint32_t> mb(idx->d);
+    std::vector<float> sc(2 * (size_t)idx->d);
+    uint32_t* maxbits = (uint32_t*)idx->trav_scale;
+    Timer t(idx, &ms);
+    e = launch_fill_u32(maxbits, 0u, idx->d, idx->stream);
+    if (e == hipSuccess) e = launch_col_absmax((const float*)idx->rows, idx->n, idx->ad, idx->d, maxbits, idx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(mb.data(), maxbits, idx->d * 4, hipMemcpyDeviceToHost, idx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(idx->stream);
+    if (e == hipSuccess) {
+      for (uint32_t j = 0; j < idx->d; j++) {
+        float m;
+        std::memcpy(&m, &mb[j], 4);
+        const bool ok = std::isfinite(m) && m >= 1e-30f;
+        sc[j] = ok ? m / 240.0f : 1.0f;
+        sc[idx->d + j] = ok ? 240.0f / m : 1.0f;
+      }
+      e = hipMemcpyAsync(idx->trav_scale, sc.data(), sc.size() * 4, hipMemcpyHostToDevice, idx->stream);
+    }
+    if (e == hipSuccess)
+      e = launch_make_fp8((const float*)idx->rows, idx->n, idx->ad, idx->d, idx->trav_scale + idx->d, rows, stride,
+                          idx->stream);
+    t.stop();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(idx->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(rows);
+    return fail(idx, DDG_EDEVICE, std::string("traversal copy: ") + hipGetErrorString(e));
+  }
+  idx->trav_rows[trav] = rows;
+  idx->trav_stride[trav] = stride;
+  if (seconds) *seconds = ms / 1000.0;
+  return DDG_OK;
+}
+
 // ====================================================================== ABI
 extern "C" {
 
@@ -495,6 +594,9 @@ void ddg_destroy(ddg_index* idx) {
     if (b.p) hipFree(b.p);
   if (idx->rows) hipFree(idx->rows);
   if (idx->graph) hipFree(idx->graph);
+  for (void* p : idx->trav_rows)
+    if (p) hipFree(p);
+  if (idx->trav_scale) hipFree(idx->trav_scale);
   if (idx->ev0) hipEventDestroy(idx->ev0);
   if (idx->ev1) hipEventDestroy(idx->ev1);
   if (idx->stream) hipStreamDestroy(idx->stream);
@@ -663,7 +765,10 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
                              uint64_t* dist_cmps, uint32_t* visited) {
   SearchArgs a{};
   Launch l{};
-  if (!search_layout(idx, beam_max, nstarts, a, l))
+  const int trav = (int)qp->trav;
+  ddg_status pst = prepare_trav(idx, trav, nullptr);
+  if (pst != DDG_OK) return pst;
+  if (!search_layout(idx, beam_max, nstarts, trav, a, l))
     return fail(idx, DDG_EINVAL, "beam width too large for the LDS-resident visited filter (max 256)");
   ENSURE(B_FLAGS, 64);
   uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
@@ -693,10 +798,11 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   a.degree_limit = qp->degree_limit ? std::min(qp->degree_limit, idx->R) : idx->R;
   a.build_mode = 0;
   a.cut_trim = idx->metric == DDG_L2 ? 1 : 0;
+  a.rerank = qp->rerank;
   l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (nq + l.waves_per_block - 1) / l.waves_per_block);
   if (l.blocks == 0) return DDG_OK;
   Timer t(idx, &idx->prof[0]);
-  HIPCHK(launch_search(idx->dtype, idx->metric, a, l, idx->stream));
+  HIPCHK(launch_search(idx->dtype, idx->metric, trav, a, l, idx->stream));
   t.stop();
   return DDG_OK;
 }
@@ -707,7 +813,12 @@ ddg_status ddg_search(ddg_index* idx, const void* queries, uint64_t nq, const ui
                       uint32_t* visited) {
   if (!idx || !qp || (nq && (!queries || !out_ids)) || !starts) return fail(idx, DDG_EINVAL, "NULL argument");
   if (nstarts == 0 || nstarts > 64) return fail(idx, DDG_EINVAL, "nstarts must be in [1, 64]");
-  if (qp->trav != DDG_TRAV_NATIVE) return fail(idx, DDG_EINVAL, "reduced-precision traversal needs ddg_search_lowp");
+  if (qp->trav != DDG_TRAV_NATIVE && qp->trav != DDG_TRAV_BF16 && qp->trav != DDG_TRAV_FP8E4M3)
+    return fail(idx, DDG_EINVAL, "unknown traversal mode");
+  if (qp->trav != DDG_TRAV_NATIVE && idx->dtype != DDG_F32)
+    return fail(idx, DDG_EINVAL, "reduced-precision traversal needs a float32 index");
+  if (qp->trav != DDG_TRAV_NATIVE && qp->rerank != 0 && qp->rerank < qp->k)
+    return fail(idx, DDG_EINVAL, "rerank must be 0 (all) or at least k");
   for (uint32_t i = 0; i < nstarts; i++)
     if (starts[i] >= idx->n) return fail(idx, DDG_ERANGE, "start point out of range");
   uint32_t beam_max = qp->beam;
@@ -760,11 +871,59 @@ ddg_status ddg_search(ddg_index* idx, const void* queries, uint64_t nq, const ui
   HIPCHK(hipStreamSynchronize(idx->stream));
   double tc = 0, tv = 0;
   for (uint64_t i = 0; i < nq; i++) { tc += (double)cm[i]; tv += (double)vs[i]; }
-  idx->prof[6] = tc * idx->d * idx->es + tv * 4.0 * (1 + idx->R) + (double)nq * idx->d * idx->es;
+  const double s_trav = trav_esize(idx, (int)qp->trav);
+  idx->prof[6] = tc * idx->d * s_trav + tv * 4.0 * (1 + idx->R) + (double)nq * idx->d * idx->es;
+  if (qp->trav != DDG_TRAV_NATIVE) {  // the FP32 re-rank reads up to min(rerank, beam) full rows per query
+    const uint32_t nr = (qp->rerank == 0 || qp->rerank > beam_max) ? beam_max : qp->rerank;
+    idx->prof[6] += (double)nq * nr * idx->d * 4.0;
+  }
   idx->prof[7] = tc;
   idx->prof[8] = tv;
   if (dist_cmps) std::memcpy(dist_cmps, cm.data(), nq * 8);
   if (visited) std::memcpy(visited, vs.data(), nq * 4);
+  return DDG_OK;
+}
+
+ddg_status ddg_prepare_traversal(ddg_index* idx, ddg_trav trav, double* seconds) {
+  if (!idx) return fail(idx, DDG_EINVAL, "NULL argument");
+  return prepare_trav(idx, (int)trav, seconds);
+}
+
+ddg_status ddg_distances(ddg_index* idx, ddg_trav trav, const void* queries, uint64_t nq, const uint32_t* ids,
+                         uint32_t m, float* out) {
+  if (!idx || (nq && m && (!queries || !ids || !out))) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (trav != DDG_TRAV_NATIVE && trav != DDG_TRAV_BF16 && trav != DDG_TRAV_FP8E4M3)
+    return fail(idx, DDG_EINVAL, "unknown traversal mode");
+  if (nq == 0 || m == 0) return DDG_OK;
+  if (nq > (1ull << 31)) return fail(idx, DDG_EINVAL, "too many queries");
+  for (uint64_t i = 0; i < nq * m; i++)
+    if (ids[i] >= idx->n) return fail(idx, DDG_ERANGE, "point id out of range");
+  ddg_status st = prepare_trav(idx, (int)trav, nullptr);
+  if (st != DDG_OK) return st;
+  SearchArgs a{};
+  const uint32_t qbytes = trav_layout(idx, (int)trav, a);
+  const size_t row_bytes = (size_t)idx->ad * idx->es;
+  ENSURE(B_QUERIES, nq * row_bytes);
+  ENSURE(B_VIS_IDS, nq * (size_t)m * 4);
+  ENSURE(B_VIS_D, nq * (size_t)m * 4);
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_QUERIES), queries, nq * row_bytes, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_VIS_IDS), ids, nq * (size_t)m * 4, hipMemcpyHostToDevice, idx->stream));
+  a.base = idx->rows;
+  a.queries = buf<void>(idx, B_QUERIES);
+  a.n = idx->n;
+  a.nq = nq;
+  a.d = idx->d;
+  a.ad = idx->ad;
+  a.lds_per_wave = qbytes;
+  Launch l{};
+  l.waves_per_block = 4;
+  l.lds_bytes = 4 * qbytes;
+  l.blocks = (uint32_t)((nq + 3) / 4);
+  Timer t(idx, &idx->prof[0]);
+  HIPCHK(launch_trav_dist(idx->dtype, idx->metric, (int)trav, a, buf<uint32_t>(idx, B_VIS_IDS), m,
+                          buf<float>(idx, B_VIS_D), l, idx->stream));
+  t.stop();
+  HIPCHK(hipMemcpy(out, buf<void>(idx, B_VIS_D), nq * (size_t)m * 4, hipMemcpyDeviceToHost));
   return DDG_OK;
 }
 

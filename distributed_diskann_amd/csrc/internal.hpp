@@ -29,8 +29,15 @@ struct SearchArgs {
   double cut;
   uint32_t d, ad, R, nstarts, k, beam, degree_limit, vcap;
   uint32_t build_mode, cut_trim;
-  // per-wave LDS layout (bytes)
-  uint32_t lds_per_wave, off_table, off_f0, off_f1, off_keep, off_cand, off_sorted, off_bkt;
+  // reduced-precision traversal (float32 indices): the copy, its row stride
+  // in bytes and 16-byte chunks per row, FP8 per-dimension scales, and the
+  // number of final beam entries re-ranked in FP32 (0 = all)
+  const void* trav_rows;
+  const float* trav_scale;
+  uint32_t trav_stride, trav_chunks, rerank;
+  // per-wave LDS layout (bytes); the staged query takes qrow_bytes at offset
+  // 0, its FP8 factors another qrow_bytes at off_qaux
+  uint32_t lds_per_wave, qrow_bytes, off_qaux, off_table, off_f0, off_f1, off_keep, off_cand, off_sorted, off_bkt;
   uint32_t fcap, rcap;
 };
 
@@ -62,7 +69,18 @@ struct Launch {
 };
 
 // launchers (return hipSuccess or an error code)
-hipError_t launch_search(int dtype, int metric, const SearchArgs& a, const Launch& l, hipStream_t s);
+hipError_t launch_search(int dtype, int metric, int trav, const SearchArgs& a, const Launch& l, hipStream_t s);
+// distances of query q to ids[q*m .. q*m+m) as the traversal computes them;
+// one wave per query, `a` carries the rows, copies and LDS layout
+hipError_t launch_trav_dist(int dtype, int metric, int trav, const SearchArgs& a, const uint32_t* ids, uint32_t m,
+                            float* out, const Launch& l, hipStream_t s);
+// reduced traversal copies of float32 rows (trav.hip)
+hipError_t launch_col_absmax(const float* rows, uint64_t n, uint32_t ad, uint32_t d, uint32_t* maxbits,
+                             hipStream_t s);
+hipError_t launch_make_bf16(const float* rows, uint64_t n, uint32_t ad, uint32_t d, void* out, uint32_t stride,
+                            hipStream_t s);
+hipError_t launch_make_fp8(const float* rows, uint64_t n, uint32_t ad, uint32_t d, const float* inv_scale, void* out,
+                           uint32_t stride, hipStream_t s);
 hipError_t launch_prune(int dtype, int metric, const PruneArgs& a, const Launch& l, hipStream_t s);
 
 // edges.hip

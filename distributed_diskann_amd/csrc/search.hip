@@ -27,18 +27,19 @@
 //    cut >= 1 (the host rejects cut < 1 with k > 0).
 #include "device.hpp"
 #include "internal.hpp"
+#include "trav.hpp"
 
 namespace ddg {
 
 constexpr uint32_t kBuckets = 1024;  // slot buckets for the lane-conflict check
 
-template <typename T, int METRIC>
+template <typename T, int METRIC, int TRAV>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  using D = Dist<T, METRIC>;
+  using P = TravDist<T, METRIC, TRAV>;
   const uint32_t lane = lane_id();
   unsigned char* my = smem + (threadIdx.x >> 6) * a.lds_per_wave;
-  T* qv = (T*)my;
+  unsigned char* ql = my;  // staged query
   uint32_t* table = (uint32_t*)(my + a.off_table);
   float* Fd0 = (float*)(my + a.off_f0);
   uint32_t* Fi0 = (uint32_t*)(Fd0 + a.fcap);
@@ -51,8 +52,6 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   float* Sd = (float*)(my + a.off_sorted);
   uint32_t* Si = (uint32_t*)(Sd + a.rcap);
 
-  const uint32_t row_bytes = a.ad * (uint32_t)sizeof(T);
-  const uint32_t nchunks = row_bytes >> 4;
   const uint64_t R1 = (uint64_t)a.R + 1;
   const T* base = (const T*)a.base;
 
@@ -72,10 +71,10 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       self_id = a.qids ? (int64_t)a.qids[q] : (int64_t)q;
       qsrc = (const T*)a.queries + (uint64_t)q * a.ad;
     }
-    stage_row(qv, qsrc, row_bytes);
+    stage_traversal_query<T, METRIC, TRAV>(a, ql, qsrc);
     for (uint32_t i = lane; i <= mask; i += kWave) table[i] = 0xFFFFFFFFu;
     wave_sync();
-    const typename D::Q qs = D::prep(qv, nchunks);
+    const typename P::Q qs = P::prep(a, ql);
 
     // starting frontier, sorted by (distance, id)
     const uint32_t ns = a.nstarts;
@@ -84,7 +83,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       uint32_t sid = 0;
       if (lane < ns) {
         sid = a.starts[lane];
-        sd = D::dist(base + (uint64_t)sid * a.ad, qv, qs, a.d, nchunks);
+        sd = P::dist(a, sid, ql, qs);
         Cd[lane] = sd;
         Ci[lane] = sid;
       }
@@ -214,7 +213,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         float dd = 0.f;
         if (v) {
           id = Kid[j];
-          dd = D::dist(base + (uint64_t)id * a.ad, qv, qs, a.d, nchunks);
+          dd = P::dist(a, id, ql, qs);
         }
         const bool pass = v && !(dd >= cutoff);
         const uint64_t pm = ballot(pass);
@@ -333,7 +332,39 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       }
     }
 
-    if (!a.build_mode) {
+    if constexpr (TRAV != TRAV_NATIVE) {
+      // FP32 re-rank of the first nr beam entries against the full rows, in
+      // the reference's arithmetic, then the first k by (distance, id)
+      const uint32_t* FI = cur ? Fi1 : Fi0;
+      float* XD = cur ? Fd0 : Fd1;
+      uint32_t* XI = cur ? Fi0 : Fi1;
+      const uint32_t nr = (a.rerank == 0 || a.rerank > fsz) ? fsz : a.rerank;
+      for (uint32_t j = lane; j < nr; j += kWave) {
+        const uint32_t id = FI[j] & kIdMask;
+        XD[j] = Dist<float, METRIC>::dist((const float*)a.base + (uint64_t)id * a.ad, (const float*)ql,
+                                          typename Dist<float, METRIC>::Q{0}, a.d, 0u);
+        XI[j] = id;
+      }
+      wave_sync();
+      for (uint32_t j = lane; j < nr; j += kWave) {
+        const float dj = XD[j];
+        const uint32_t ij = XI[j];
+        uint32_t rank = 0;
+        for (uint32_t t = 0; t < nr; t++) {
+          const float dt = XD[t];
+          const uint32_t it = XI[t];
+          rank += (less_di(dt, it, dj, ij) || (dt == dj && it == ij && t < j)) ? 1u : 0u;
+        }
+        if (rank < a.k) {
+          a.out_ids[(uint64_t)q * a.k + rank] = ij;
+          if (a.out_dists) a.out_dists[(uint64_t)q * a.k + rank] = dj;
+        }
+      }
+      for (uint32_t j = nr + lane; j < a.k; j += kWave) {
+        a.out_ids[(uint64_t)q * a.k + j] = 0xFFFFFFFFu;
+        if (a.out_dists) a.out_dists[(uint64_t)q * a.k + j] = __builtin_inff();
+      }
+    } else if (!a.build_mode) {
       const float* FD = cur ? Fd1 : Fd0;
       const uint32_t* FI = cur ? Fi1 : Fi0;
       for (uint32_t j = lane; j < a.k; j += kWave) {
@@ -350,17 +381,69 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   }
 }
 
-template <typename T, int M>
+// Distances of query q (wave q) to ids[q*m .. q*m+m), computed exactly as the
+// search's traversal computes them; ids >= n give NaN.
+template <typename T, int METRIC, int TRAV>
+__global__ __launch_bounds__(256) void k_trav_dist(SearchArgs a, const uint32_t* ids, uint32_t m, float* out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using P = TravDist<T, METRIC, TRAV>;
+  unsigned char* ql = smem + (threadIdx.x >> 6) * a.lds_per_wave;
+  const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (q >= a.nq) return;  // whole waves leave together
+  stage_traversal_query<T, METRIC, TRAV>(a, ql, (const T*)a.queries + q * a.ad);
+  wave_sync();
+  const typename P::Q qs = P::prep(a, ql);
+  for (uint32_t j = lane_id(); j < m; j += kWave) {
+    const uint32_t id = ids[q * m + j];
+    out[q * m + j] = id < a.n ? P::dist(a, id, ql, qs) : __builtin_nanf("");
+  }
+}
+
+template <typename T, int M, int TR>
 static hipError_t launch_t(const SearchArgs& a, const Launch& l, hipStream_t s) {
-  hipFuncSetAttribute((const void*)&k_search<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes);
-  hipLaunchKernelGGL((k_search<T, M>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
+  hipFuncSetAttribute((const void*)&k_search<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes);
+  hipLaunchKernelGGL((k_search<T, M, TR>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_search(int dtype, int metric, const SearchArgs& a, const Launch& l, hipStream_t s) {
-  if (dtype == U8) return metric == L2 ? launch_t<uint8_t, L2>(a, l, s) : launch_t<uint8_t, MIPS>(a, l, s);
-  if (dtype == I8) return metric == L2 ? launch_t<int8_t, L2>(a, l, s) : launch_t<int8_t, MIPS>(a, l, s);
-  return metric == L2 ? launch_t<float, L2>(a, l, s) : launch_t<float, MIPS>(a, l, s);
+template <typename T, int M, int TR>
+static hipError_t launch_d(const SearchArgs& a, const uint32_t* ids, uint32_t m, float* out, const Launch& l,
+                           hipStream_t s) {
+  hipFuncSetAttribute((const void*)&k_trav_dist<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)l.lds_bytes);
+  hipLaunchKernelGGL((k_trav_dist<T, M, TR>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a,
+                     ids, m, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_search(int dtype, int metric, int trav, const SearchArgs& a, const Launch& l, hipStream_t s) {
+  if (trav == TRAV_BF16)
+    return metric == L2 ? launch_t<float, L2, TRAV_BF16>(a, l, s) : launch_t<float, MIPS, TRAV_BF16>(a, l, s);
+  if (trav == TRAV_FP8)
+    return metric == L2 ? launch_t<float, L2, TRAV_FP8>(a, l, s) : launch_t<float, MIPS, TRAV_FP8>(a, l, s);
+  if (dtype == U8)
+    return metric == L2 ? launch_t<uint8_t, L2, TRAV_NATIVE>(a, l, s) : launch_t<uint8_t, MIPS, TRAV_NATIVE>(a, l, s);
+  if (dtype == I8)
+    return metric == L2 ? launch_t<int8_t, L2, TRAV_NATIVE>(a, l, s) : launch_t<int8_t, MIPS, TRAV_NATIVE>(a, l, s);
+  return metric == L2 ? launch_t<float, L2, TRAV_NATIVE>(a, l, s) : launch_t<float, MIPS, TRAV_NATIVE>(a, l, s);
+}
+
+hipError_t launch_trav_dist(int dtype, int metric, int trav, const SearchArgs& a, const uint32_t* ids, uint32_t m,
+                            float* out, const Launch& l, hipStream_t s) {
+  if (trav == TRAV_BF16)
+    return metric == L2 ? launch_d<float, L2, TRAV_BF16>(a, ids, m, out, l, s)
+                        : launch_d<float, MIPS, TRAV_BF16>(a, ids, m, out, l, s);
+  if (trav == TRAV_FP8)
+    return metric == L2 ? launch_d<float, L2, TRAV_FP8>(a, ids, m, out, l, s)
+                        : launch_d<float, MIPS, TRAV_FP8>(a, ids, m, out, l, s);
+  if (dtype == U8)
+    return metric == L2 ? launch_d<uint8_t, L2, TRAV_NATIVE>(a, ids, m, out, l, s)
+                        : launch_d<uint8_t, MIPS, TRAV_NATIVE>(a, ids, m, out, l, s);
+  if (dtype == I8)
+    return metric == L2 ? launch_d<int8_t, L2, TRAV_NATIVE>(a, ids, m, out, l, s)
+                        : launch_d<int8_t, MIPS, TRAV_NATIVE>(a, ids, m, out, l, s);
+  return metric == L2 ? launch_d<float, L2, TRAV_NATIVE>(a, ids, m, out, l, s)
+                      : launch_d<float, MIPS, TRAV_NATIVE>(a, ids, m, out, l, s);
 }
 
 }  // namespace ddg

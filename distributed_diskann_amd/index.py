@@ -124,17 +124,36 @@ class Index:
         return ids, dists, cmps, vis
 
     def search_device(self, Q, k, beam, out_ids, out_dists=None, cmps=None, visited=None, cut=1.35, limit=0,
-                      degree_limit=0, starts=None, per_query_beam=None, beam_max=None, query_ids=None):
+                      degree_limit=0, starts=None, per_query_beam=None, beam_max=None, query_ids=None,
+                      trav="native", rerank=0):
         """Search over torch CUDA tensors (device-resident inputs and outputs)."""
         if starts is None:
             raise ValueError("search_device needs a device tensor of start points")
-        qp = QueryParamsC(k, beam, cut, limit, degree_limit, 0, 0)
+        qp = QueryParamsC(k, beam, cut, limit, degree_limit, TRAV[trav], rerank)
         p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
         beam_max = beam if beam_max is None else beam_max
         nst = 1 if starts is None else starts.numel()
         self._chk(lib().ddg_search_device(self.h, p(Q), Q.shape[0], p(query_ids), p(starts), nst, C.byref(qp),
                                           p(per_query_beam), beam_max, p(out_ids), p(out_dists), p(cmps),
                                           p(visited)))
+
+    def prepare_traversal(self, trav):
+        """Builds the BF16 / FP8 traversal copy; returns its device seconds."""
+        sec = C.c_double(0.0)
+        self._chk(lib().ddg_prepare_traversal(self.h, TRAV[trav], C.byref(sec)))
+        return sec.value
+
+    def distances(self, Q: np.ndarray, ids, trav="native"):
+        """Distances of query i to ids[i, j] in the arithmetic of `trav`."""
+        Q = np.ascontiguousarray(Q)
+        if Q.dtype != self.rows.dtype or Q.shape[1] != self.ad:
+            raise ValueError("queries must match the base dtype and aligned dimension")
+        ids = np.ascontiguousarray(np.asarray(ids, _U32))
+        if ids.ndim != 2 or ids.shape[0] != Q.shape[0]:
+            raise ValueError("ids must be (nq, m)")
+        out = np.empty(ids.shape, np.float32)
+        self._chk(lib().ddg_distances(self.h, TRAV[trav], _c(Q), Q.shape[0], _c(ids), ids.shape[1], _c(out)))
+        return out
 
     def insert_search(self, ids, L, start_point=0, vcap=None):
         ids = np.ascontiguousarray(np.asarray(ids, _U32))

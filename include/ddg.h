@@ -43,10 +43,15 @@ typedef enum { DDG_U8 = 0, DDG_I8 = 1, DDG_F32 = 2 } ddg_dtype;
  * (mips_point.h:68-109, not metric: no cut trimming in search). */
 typedef enum { DDG_L2 = 0, DDG_MIPS = 1 } ddg_metric;
 
-/* Vector copy the search traverses with (float32 indices only):
- * NATIVE = the f32 rows, exact reference arithmetic; BF16 / FP8E4M3 = reduced
- * copies with per-dimension scales (FP8) and FP32 accumulation, followed by
- * an FP32 re-rank of the final beam. */
+/* Vector copy the search traverses with:
+ * NATIVE  = the index's own rows, the reference's arithmetic (any dtype);
+ * BF16    = a BF16 copy of float32 rows (round to nearest even), FP32 query
+ *           and FP32 accumulation;
+ * FP8E4M3 = an FP8-e4m3 copy of float32 rows with one scale per dimension,
+ *           x_ij ~ s_j * e_ij, s_j = max_i |x_ij| / 240, FP32 accumulation.
+ * Both reduced modes end with an FP32 re-rank of the final beam against the
+ * full rows (the reference's own distance), so returned distances are exact
+ * FP32 distances. */
 typedef enum { DDG_TRAV_NATIVE = 0, DDG_TRAV_BF16 = 1, DDG_TRAV_FP8E4M3 = 2 } ddg_trav;
 
 /* BuildParams (types.h:103-142) + the batch_insert schedule knobs
@@ -70,7 +75,8 @@ typedef struct {
   uint64_t limit;        /* max visited vertices (0 = n)                     */
   uint32_t degree_limit; /* 0 = R                                            */
   ddg_trav trav;
-  uint32_t rerank;       /* beam entries re-ranked in FP32 (0 = all)         */
+  uint32_t rerank;       /* reduced trav: beam entries re-ranked in FP32
+                            (0 = the whole final beam, otherwise >= k)      */
 } ddg_query_params;
 
 /* Replaces PointRange(char*) + Graph(R, n) (point_range.h:61-97,
@@ -144,6 +150,20 @@ ddg_status ddg_ground_truth(ddg_index* idx, const void* queries /* nq*aligned_d 
 
 /* index.h:133-136: sort every adjacency list by (distance to its owner, id). */
 ddg_status ddg_sort_rows(ddg_index* idx);
+
+/* Builds the reduced traversal copy of a float32 index (BF16 or FP8E4M3; a
+ * no-op for NATIVE or when it exists).  ddg_search builds it on first use;
+ * calling this first keeps the conversion out of a timed search.  seconds
+ * (nullable) receives the device time of the conversion.  No reference
+ * counterpart: the reference traverses its float32 rows only. */
+ddg_status ddg_prepare_traversal(ddg_index* idx, ddg_trav trav, double* seconds);
+
+/* Point::distance (euclidian_point.h:44-65, mips_point.h:44-66) batched, in
+ * the arithmetic of traversal mode trav: out[i*m + j] = distance of query i
+ * (nq*aligned_d rows) to point ids[i*m + j].  NATIVE gives the reference's
+ * bits; BF16 / FP8E4M3 give the values the reduced traversal ranks by. */
+ddg_status ddg_distances(ddg_index* idx, ddg_trav trav, const void* queries, uint64_t nq, const uint32_t* ids,
+                         uint32_t m, float* out);
 
 /* Size / layout queries. */
 uint64_t ddg_size(const ddg_index* idx);
