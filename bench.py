@@ -16,6 +16,13 @@ FP32, BF16 and FP8-e4m3 traversal (re-rank of the final 100 in FP32), with
 each mode's distance error against FP64.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--lowp-n N]
+    python bench.py --workload {i8,u8,mips} [--n N] [--nq NQ] [--lowp-n 0]
+
+Other workloads run the same protocol (build, FP64 ground truth, Ls sweep,
+timed device-resident search; float32 workloads also time BF16 and FP8
+traversal with FP32 re-rank).  Above --cpu-full-max points the CPU baseline
+is a bounded sample: the oracle builds the first --cpu-build-n points, and
+searches --cpu-nq queries over the GPU-built graph at the chosen Ls.
 
 Prints one JSON line.
 """
@@ -161,8 +168,13 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="s")
+    ap.add_argument("--n", type=int, default=0, help="base points (0 = the workload's own size)")
+    ap.add_argument("--nq", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-full-max", type=int, default=200_000)
+    ap.add_argument("--cpu-build-n", type=int, default=100_000)
+    ap.add_argument("--cpu-nq", type=int, default=1000)
     ap.add_argument("--lowp-n", type=int, default=1_000_000, help="base points of the low-precision workload "
                                                                   "(0 skips it)")
     ap.add_argument("--lowp-nq", type=int, default=10_000)
@@ -177,9 +189,11 @@ def main():
     from distributed_diskann_amd import Index, datasets
     from distributed_diskann_amd.index import recall_with_ties
 
-    X, Q, w = datasets.generate(args.workload)
-    nq = Q.shape[0]
-    log(f"[bench] workload {w.name}: n={w.n} d={w.d} {w.dtype} {w.metric} R={w.R} L={w.L} alpha={w.alpha} nq={nq}")
+    t0 = time.perf_counter()
+    X, Q, w = datasets.generate(args.workload, n=args.n or None, nq=args.nq or None)
+    n, nq = X.shape[0], Q.shape[0]
+    log(f"[bench] workload {w.name}: n={n} d={w.d} {w.dtype} {w.metric} R={w.R} L={w.L} alpha={w.alpha} nq={nq} "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
     idx = Index(X, w.d, w.metric, w.R)
     t0 = time.perf_counter()
     build_s = idx.build(R=w.R, L=w.L, alpha=w.alpha)
@@ -211,46 +225,87 @@ def main():
     mixed_ms, mixed_kern = ds.timed(chosen, max(1, args.steps // 4), 1, pqb=pqb, bmax=int(beams.max()))
     mixed_rec = recall_with_ties(ds.result_ids(), gt_ids, gt_d, 10)
     log(f"[bench] per-query beams: {mixed_ms:.3f} ms/step, recall {mixed_rec:.4f}")
+
+    # reduced-precision traversal of the same index (float32 workloads)
+    modes = None
+    if w.dtype == "float32":
+        modes = {}
+        for mode in ("bf16", "fp8"):
+            prep = idx.prepare_traversal(mode)
+            Ls_m, r_m = sweep(ds, gt_ids, gt_d, mode, 100)
+            wall_m, kern_m = ds.timed(Ls_m, max(3, args.steps // 4), 1, mode, 100)
+            cm_m, vs_m = ds.counters()
+            B_m = alg_bytes(cm_m, vs_m, nq, w.d, ESIZE[mode], avg_deg, min(100, Ls_m))
+            modes[mode] = {"Ls": Ls_m, "recall": r_m, "qps": nq / (wall_m / 1000.0), "kernel_ms": kern_m,
+                           "alg_GBps": B_m / (kern_m / 1000.0) / 1e9, "prepare_s": prep}
+            log(f"[bench] {mode}: Ls={Ls_m} recall {r_m:.4f} QPS {nq / (wall_m / 1000.0):.0f} kernel {kern_m:.3f} ms")
     del ds
 
-    lowp = low_precision(torch, args) if args.lowp_n > 0 else None
+    lowp = low_precision(torch, args) if (args.lowp_n > 0 and args.workload == "s") else None
 
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        P = O.Points(X, w.d, w.metric)
-        t0 = time.perf_counter()
-        Gc = O.build(P, w.R, w.L, w.alpha, tie_mode=O.TIE_REFERENCE, nthreads=threads)
-        cpu_build = time.perf_counter() - t0
-        O.search_all(P, Gc, Q, 10, chosen, cut=1.35, nthreads=threads)  # warm-up (check_nn_recall.h:62-66)
-        t0 = time.perf_counter()
-        ci, _, _, _ = O.search_all(P, Gc, Q, 10, chosen, cut=1.35, nthreads=threads)
-        cpu_search = time.perf_counter() - t0
-        cpu_rec = recall_with_ties(ci, gt_ids, gt_d, 10)
-        cpu = {"value": nq / cpu_search, "unit": "queries/s", "cores": threads, "kind": "port",
-               "sample": f"full workload: build of all {w.n} points and {nq} queries at Ls={chosen} "
-                         f"(CPU recall {cpu_rec:.4f})",
-               "build_seconds": cpu_build}
-        log(f"[bench] CPU ({threads} threads): build {cpu_build:.2f} s, QPS {nq / cpu_search:.0f}")
+        if n <= args.cpu_full_max:
+            P = O.Points(X, w.d, w.metric)
+            t0 = time.perf_counter()
+            Gc = O.build(P, w.R, w.L, w.alpha, tie_mode=O.TIE_REFERENCE, nthreads=threads)
+            cpu_build = time.perf_counter() - t0
+            O.search_all(P, Gc, Q, 10, chosen, cut=1.35, nthreads=threads)  # warm-up (check_nn_recall.h:62-66)
+            t0 = time.perf_counter()
+            ci, _, _, _ = O.search_all(P, Gc, Q, 10, chosen, cut=1.35, nthreads=threads)
+            cpu_search = time.perf_counter() - t0
+            cpu_rec = recall_with_ties(ci, gt_ids, gt_d, 10)
+            cpu = {"value": nq / cpu_search, "unit": "queries/s", "cores": threads, "kind": "port",
+                   "sample": f"full workload: build of all {n} points and {nq} queries at Ls={chosen} "
+                             f"(CPU recall {cpu_rec:.4f})",
+                   "build_seconds": cpu_build}
+            log(f"[bench] CPU ({threads} threads): build {cpu_build:.2f} s, QPS {nq / cpu_search:.0f}")
+        else:
+            # bounded sample: a build of the first cpu_build_n points, and cpu_nq
+            # queries searched over the GPU-built graph (the same graph, so the
+            # same visits) at the chosen Ls
+            m = min(args.cpu_build_n, n)
+            Ps = O.Points(np.ascontiguousarray(X[:m]), w.d, w.metric)
+            t0 = time.perf_counter()
+            O.build(Ps, w.R, w.L, w.alpha, tie_mode=O.TIE_REFERENCE, nthreads=threads)
+            cpu_build_m = time.perf_counter() - t0
+            del Ps
+            P = O.Points(X, w.d, w.metric)
+            Gg = O.Graph(G)
+            qs = np.ascontiguousarray(Q[:args.cpu_nq])
+            O.search_all(P, Gg, qs[:100], 10, chosen, cut=1.35, nthreads=threads)  # warm-up
+            t0 = time.perf_counter()
+            O.search_all(P, Gg, qs, 10, chosen, cut=1.35, nthreads=threads)
+            cpu_search = time.perf_counter() - t0
+            cpu = {"value": qs.shape[0] / cpu_search, "unit": "queries/s", "cores": threads, "kind": "port",
+                   "sample": f"{qs.shape[0]} of {nq} queries at Ls={chosen} over the GPU-built {n}-point graph; "
+                             f"build of the first {m} points only",
+                   "build_seconds_sample": cpu_build_m, "build_points_per_s_sample": m / cpu_build_m}
+            log(f"[bench] CPU ({threads} threads): build of {m} points {cpu_build_m:.2f} s "
+                f"({m / cpu_build_m:.0f} points/s), QPS {qs.shape[0] / cpu_search:.0f}")
+            del P, Gg
 
     out = {
         "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32",
-        "data": f"synthetic: {w.n}x{w.d} float32 N(0,1) seed {w.base_seed}, {nq} queries seed {w.query_seed}",
-        "config": {"workload": f"{w.n}x{w.d} f32 L2, R={w.R} L={w.L} alpha={w.alpha}, k=10, Ls={chosen}",
-                   "n": w.n, "d": w.d, "nq": nq, "R": w.R, "L": w.L, "alpha": w.alpha, "k": 10, "Ls": chosen},
+        "vs_baseline": None, "dtype": {"float32": "f32", "uint8": "u8", "int8": "i8"}[w.dtype],
+        "data": f"synthetic {w.name}: {n}x{w.d} {w.dtype} seed {w.base_seed}, {nq} queries seed {w.query_seed}",
+        "config": {"workload": f"{n}x{w.d} {w.dtype} {w.metric}, R={w.R} L={w.L} alpha={w.alpha}, k=10, Ls={chosen}",
+                   "n": n, "d": w.d, "nq": nq, "R": w.R, "L": w.L, "alpha": w.alpha, "k": 10, "Ls": chosen},
         "recall": rec,
         "build_seconds": build_s,
         "build_profile_ms": {k: v for k, v in bprof.items() if k.endswith("_ms")},
         "per_query_beam": {"qps": nq / (mixed_ms / 1000.0), "kernel_ms": mixed_kern, "recall": mixed_rec,
                            "beams": [10, 20, 50, 100, 200]},
+        "reduced_traversal": modes,
         "low_precision": lowp,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_search<float,L2,native>", "kernel_avg_ms": kern_avg_ms,
-                     "note": "12.8 MB of vectors: the working set is cache-resident"},
+                     "kernel": f"k_search<{w.dtype},{w.metric},native>", "kernel_avg_ms": kern_avg_ms,
+                     "note": (f"{X.nbytes / 1e6:.1f} MB of vectors" +
+                              (": the working set is cache-resident" if X.nbytes < 256e6 else ""))},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

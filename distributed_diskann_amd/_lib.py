@@ -52,6 +52,14 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run `python -m distributed_diskann_amd.build` "
                           "(there is no CPU fallback)")
+    # torch's wheel carries its own libamdhip64 / libhsa-runtime64 under the
+    # same sonames as /opt/rocm's.  Loaded after libddg.so, it would be a
+    # second HIP runtime in the process (torch then may find no device);
+    # loaded first, libddg.so binds to it and the process has one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     P, vp = C.POINTER, C.c_void_p
     u32, u64, i32, dbl = C.c_uint32, C.c_uint64, C.c_int, C.c_double

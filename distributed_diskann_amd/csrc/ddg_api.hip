@@ -25,13 +25,13 @@ struct Buf {
 };
 
 struct Scratch {
-  Buf b[32];
+  Buf b[40];
 };
 
 enum BufId {
   B_QUERIES, B_STARTS, B_QBEAM, B_QIDS, B_OUT_IDS, B_OUT_D, B_CMPS, B_VISN, B_VIS_IDS, B_VIS_D, B_FLAGS,
   B_INSERTS, B_NEW_IDS, B_NEW_LEN, B_OFFS, B_KEYS, B_VALS, B_KEYS2, B_VALS2, B_TMP, B_GFLAGS, B_GOFFS, B_GKEYS,
-  B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D
+  B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D, B_GTAB
 };
 
 }  // namespace
@@ -144,11 +144,30 @@ static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
   return trav == DDG_TRAV_FP8E4M3 ? 2 * a.qrow_bytes : a.qrow_bytes;
 }
 
+// Where the visited filter of a search lives: LDS up to kLdsTableMax bytes,
+// global memory above (one slice per resident wave).  DDG_GTABLE=0 / 1
+// forces LDS / global for every launch.
+constexpr uint32_t kLdsTableMax = 8192;
+static bool global_table(uint32_t bits) {
+  static int force = -2;
+  if (force == -2) {
+    const char* e = getenv("DDG_GTABLE");
+    force = (e && (e[0] == '0' || e[0] == '1')) ? e[0] - '0' : -1;
+  }
+  if (force >= 0) return force == 1;
+  return (4u << bits) > kLdsTableMax;
+}
+
+// Fills the LDS layout and launch shape of a search; a.gtable_words > 0 asks
+// the caller for a global filter pool of blocks * waves_per_block slices.
 static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, int trav, SearchArgs& a,
                           Launch& l) {
   const uint32_t qrow = trav_layout(idx, trav, a);
   const uint32_t bits = filter_bits(beam_max);
-  const uint32_t table = 4u << bits;
+  const bool gtab = global_table(bits);
+  const uint32_t table = gtab ? 0u : 4u << bits;
+  a.gtable = nullptr;
+  a.gtable_words = gtab ? 1u << bits : 0u;
   const uint32_t fcap = round_up(std::max(beam_max, nstarts), 16);
   const uint32_t rcap = round_up(std::max<uint32_t>(idx->R, 64), 64);
   uint32_t off = 0;
@@ -173,20 +192,41 @@ static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nsta
   return true;
 }
 
+#define ENSURE_GTABLE(a, l)                                                                                 \
+  do {                                                                                                     \
+    if ((a).gtable_words) {                                                                                \
+      ENSURE(B_GTAB, (size_t)(l).blocks * (l).waves_per_block * (a).gtable_words * 4);                      \
+      (a).gtable = buf<uint32_t>(idx, B_GTAB);                                                             \
+    }                                                                                                      \
+  } while (0)
+
+// Per-wave LDS of a prune launch: p's and p*'s rows, the selection, ccap
+// candidates (distance, id, row slot), then a cache of rrows candidate rows
+// sized to keep the wave within kPruneWaveLds (no cache when fewer than 16
+// rows fit).
+constexpr uint32_t kPruneWaveLds = 24u * 1024u;
 static void prune_layout(const ddg_index* idx, uint32_t Rsel, uint32_t want_ccap, PruneArgs& a, Launch& l) {
   const uint32_t row = round_up(idx->ad * idx->es, 16);
   uint32_t ccap = 64;
   while (ccap < want_ccap && ccap < 4096) ccap <<= 1;
-  auto per_wave = [&](uint32_t c) { return 2 * row + round_up(Rsel * 4, 16) + c * 8; };
-  while (ccap > 64 && per_wave(ccap) * 2 > kLdsPerCU) ccap >>= 1;
+  auto fixed = [&](uint32_t c) { return 2 * row + round_up(Rsel * 4, 16) + round_up(c * 10, 16); };
+  while (ccap > 64 && fixed(ccap) * 2 > kLdsPerCU) ccap >>= 1;
   a.ccap = ccap;
   a.off_cand = 2 * row + round_up(Rsel * 4, 16);
-  a.lds_per_wave = round_up(per_wave(ccap), 16);
-  uint32_t wpb = std::min<uint32_t>(4, kLdsPerCU / a.lds_per_wave);
-  const uint32_t per_block = wpb * a.lds_per_wave;
-  uint32_t bpc = std::max<uint32_t>(1, std::min<uint32_t>(32 / wpb, kLdsPerCU / per_block));
+  a.off_rows = fixed(ccap);
+  uint32_t rrows = fixed(ccap) < kPruneWaveLds ? (kPruneWaveLds - fixed(ccap)) / row : 0u;
+  rrows = std::min<uint32_t>(rrows, std::min<uint32_t>(ccap, round_up(std::max<uint32_t>(want_ccap, 16), 16)));
+  a.rrows = rrows >= 16 ? rrows : 0u;
+  a.lds_per_wave = round_up(fixed(ccap) + a.rrows * row, 16);
+  // the block shape (1, 2 or 4 waves) that fits the most waves on a CU
+  uint32_t wpb = 1, bpc = 1, best = 0;
+  for (uint32_t w : {4u, 2u, 1u}) {
+    if (w * a.lds_per_wave > kLdsPerCU) continue;
+    const uint32_t b = std::min<uint32_t>(32 / w, kLdsPerCU / (w * a.lds_per_wave));
+    if (w * b > best) { best = w * b; wpb = w; bpc = b; }
+  }
   l.waves_per_block = wpb;
-  l.lds_bytes = per_block;
+  l.lds_bytes = wpb * a.lds_per_wave;
   l.blocks = idx->num_cu * bpc;
 }
 
@@ -254,6 +294,7 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
     a.build_mode = 1;
     a.cut_trim = 0;
     l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (B + l.waves_per_block - 1) / l.waves_per_block);
+    ENSURE_GTABLE(a, l);
     Timer t(idx, &idx->prof[0]);
     HIPCHK(launch_search(idx->dtype, idx->metric, DDG_TRAV_NATIVE, a, l, idx->stream));
     t.stop();
@@ -316,6 +357,7 @@ static ddg_status run_prune(ddg_index* idx, PruneArgs a, uint64_t np, uint32_t m
     lb.blocks = idx->num_cu;
     lb.lds_bytes = 4 * round_up(2 * round_up(idx->ad * idx->es, 16) + round_up(a.Rsel * 4, 16), 16);
     a.lds_per_wave = lb.lds_bytes / 4;
+    a.rrows = 0;
     Timer t2(idx, acc);
     HIPCHK(launch_prune(idx->dtype, idx->metric, a, lb, idx->stream));
     t2.stop();
@@ -801,6 +843,7 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   a.rerank = qp->rerank;
   l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (nq + l.waves_per_block - 1) / l.waves_per_block);
   if (l.blocks == 0) return DDG_OK;
+  ENSURE_GTABLE(a, l);
   Timer t(idx, &idx->prof[0]);
   HIPCHK(launch_search(idx->dtype, idx->metric, trav, a, l, idx->stream));
   t.stop();
@@ -1055,12 +1098,14 @@ ddg_status ddg_ground_truth(ddg_index* idx, const void* queries, uint64_t nq, ui
   ENSURE(B_QUERIES, nq * row_bytes);
   ENSURE(B_OUT_IDS, nq * k * 4);
   ENSURE(B_GT_D, nq * k * 8);
+  const size_t tmp_bytes = ground_truth_tmp_bytes(idx->dtype, idx->n, nq, idx->d, k);
+  ENSURE(B_TMP, tmp_bytes);
   HIPCHK(hipMemcpyAsync(buf<void>(idx, B_QUERIES), queries, nq * row_bytes, hipMemcpyHostToDevice, idx->stream));
   HIPCHK(launch_pad_rows(buf<void>(idx, B_QUERIES), nq, idx->d * idx->es, (uint32_t)row_bytes, idx->stream));
   Timer t(idx, &idx->prof[5]);
   HIPCHK(launch_ground_truth(idx->dtype, idx->metric, idx->rows, idx->n, buf<void>(idx, B_QUERIES), nq, idx->d,
-                             idx->ad, k, buf<uint32_t>(idx, B_OUT_IDS), buf<double>(idx, B_GT_D), nullptr, 0,
-                             idx->stream));
+                             idx->ad, k, buf<uint32_t>(idx, B_OUT_IDS), buf<double>(idx, B_GT_D),
+                             buf<void>(idx, B_TMP), tmp_bytes, idx->stream));
   t.stop();
   HIPCHK(hipMemcpy(ids, buf<void>(idx, B_OUT_IDS), nq * k * 4, hipMemcpyDeviceToHost));
   if (dists) HIPCHK(hipMemcpy(dists, buf<void>(idx, B_GT_D), nq * k * 8, hipMemcpyDeviceToHost));

@@ -35,6 +35,10 @@ struct SearchArgs {
   const void* trav_rows;
   const float* trav_scale;
   uint32_t trav_stride, trav_chunks, rerank;
+  // visited filter in global memory (null: in LDS), gtable_words per
+  // resident wave, indexed by block * waves_per_block + wave
+  uint32_t* gtable;
+  uint32_t gtable_words;
   // per-wave LDS layout (bytes); the staged query takes qrow_bytes at offset
   // 0, its FP8 factors another qrow_bytes at off_qaux
   uint32_t lds_per_wave, qrow_bytes, off_qaux, off_table, off_f0, off_f1, off_keep, off_cand, off_sorted, off_bkt;
@@ -62,6 +66,8 @@ struct PruneArgs {
   double alpha;
   uint32_t a_stride, d, ad, R1, Rsel, ccap, add, write_graph, append_if_fits;
   uint32_t lds_per_wave, off_cand, gcap, big_pass;
+  // LDS row cache: tasks with at most rrows candidates stage their rows at off_rows
+  uint32_t rrows, off_rows;
 };
 
 struct Launch {
@@ -107,6 +113,6 @@ hipError_t launch_pad_rows(void* rows, uint64_t n, uint32_t d_bytes, uint32_t ro
 hipError_t launch_ground_truth(int dtype, int metric, const void* base, uint64_t n, const void* queries,
                                uint64_t nq, uint32_t d, uint32_t ad, uint32_t k, uint32_t* ids, double* dists,
                                void* tmp, size_t tmp_bytes, hipStream_t s);
-size_t ground_truth_tmp_bytes(uint64_t n, uint64_t nq, uint32_t k);
+size_t ground_truth_tmp_bytes(int dtype, uint64_t n, uint64_t nq, uint32_t d, uint32_t k);
 
 }  // namespace ddg

@@ -16,6 +16,10 @@
 // A task whose candidate list exceeds the LDS capacity is queued in
 // big_tasks and re-run by a second launch (big_pass = 1) that keeps its
 // candidates in a per-wave global scratch instead.
+//
+// When a task's candidate rows fit the per-wave LDS row cache (rrows rows),
+// they are staged there once (STAGED): d(p, c) and every d(p*, p') of the
+// greedy sweep then read LDS instead of HBM, with the same arithmetic.
 #include "device.hpp"
 #include "internal.hpp"
 
@@ -23,10 +27,10 @@ namespace ddg {
 
 // One robustPrune task (index.h:63-110) with the wave: candidates, sort,
 // greedy selection, write-back.
-template <typename T, int METRIC, bool BIG>
+template <typename T, int METRIC, bool BIG, bool STAGED>
 __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint32_t p, uint32_t* grow, uint64_t a0,
                                            uint64_t alen, uint32_t gdeg, T* pv, T* sv, uint32_t* sel, float* Cd,
-                                           uint32_t* Ci) {
+                                           uint32_t* Ci, uint16_t* Cs, T* rows) {
   {
     using D = Dist<T, METRIC>;
     const uint32_t lane = lane_id();
@@ -35,27 +39,48 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
     const T* base = (const T*)a.base;
     const uint32_t M = (uint32_t)(alen + gdeg);
     stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
-    wave_sync();
-    const typename D::Q ps = D::prep(pv, nchunks);
-
     // ---- candidates: A (given or computed distances) then G[p]
-    for (uint32_t j = lane; j < M; j += kWave) {
-      uint32_t id;
-      float dd;
-      if (j < alen) {
-        id = a.a_ids[a0 + j];
-        dd = a.a_dists ? a.a_dists[a0 + j]
-                       : D::dist(base + (uint64_t)id * a.ad, pv, ps, a.d, nchunks);
-      } else {
-        id = grow[1 + (j - (uint32_t)alen)];
-        dd = D::dist(base + (uint64_t)id * a.ad, pv, ps, a.d, nchunks);
+    for (uint32_t j = lane; j < M; j += kWave)
+      Ci[j] = j < alen ? a.a_ids[a0 + j] : grow[1 + (j - (uint32_t)alen)];
+    if (BIG) __threadfence_block();
+    wave_sync();
+    if constexpr (STAGED) {  // candidate j's row to rows[j], eight 16-byte loads in flight per lane
+      const uint32_t total = M * nchunks;
+      for (uint32_t e0 = lane; e0 < total; e0 += 8u * kWave) {
+        uint4 v[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) {
+          const uint32_t e = e0 + u * kWave;
+          if (e < total) {
+            const uint32_t j = e / nchunks, c = e - j * nchunks;
+            v[u] = ((const uint4*)(base + (uint64_t)Ci[j] * a.ad))[c];
+          }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8; u++) {
+          const uint32_t e = e0 + u * kWave;
+          if (e < total) {
+            const uint32_t j = e / nchunks, c = e - j * nchunks;
+            ((uint4*)(rows + (uint64_t)j * a.ad))[c] = v[u];
+          }
+        }
       }
-      Cd[j] = dd;
-      Ci[j] = id;
+      wave_sync();
+    }
+    const typename D::Q ps = D::prep(pv, nchunks);
+    for (uint32_t j = lane; j < M; j += kWave) {
+      const uint32_t id = Ci[j];
+      const T* r = STAGED ? rows + (uint64_t)j * a.ad : base + (uint64_t)id * a.ad;
+      Cd[j] = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(r, pv, ps, a.d, nchunks);
+      if (STAGED) Cs[j] = (uint16_t)j;
     }
     uint32_t P2 = 1;
     while (P2 < M) P2 <<= 1;
-    for (uint32_t j = M + lane; j < P2; j += kWave) { Cd[j] = __builtin_inff(); Ci[j] = kPruned; }
+    for (uint32_t j = M + lane; j < P2; j += kWave) {
+      Cd[j] = __builtin_inff();
+      Ci[j] = kPruned;
+      if (STAGED) Cs[j] = 0;
+    }
     if (BIG) __threadfence_block();
     wave_sync();
 
@@ -69,7 +94,10 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
           const float dl = Cd[lo], dh = Cd[hi];
           const uint32_t il = Ci[lo], ih = Ci[hi];
           const bool sw = up ? less_di(dh, ih, dl, il) : less_di(dl, il, dh, ih);
-          if (sw) { Cd[lo] = dh; Cd[hi] = dl; Ci[lo] = ih; Ci[hi] = il; }
+          if (sw) {
+            Cd[lo] = dh; Cd[hi] = dl; Ci[lo] = ih; Ci[hi] = il;
+            if (STAGED) { const uint16_t sl = Cs[lo]; Cs[lo] = Cs[hi]; Cs[hi] = sl; }
+          }
         }
         if (BIG) __threadfence_block();
         wave_sync();
@@ -96,13 +124,19 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
       nsel++;
       idx = (uint32_t)found + 1u;
       if (nsel >= a.Rsel) break;  // the reference's remaining prunes cannot change the result
-      stage_row(sv, base + (uint64_t)pstar * a.ad, row_bytes);
-      wave_sync();
-      const typename D::Q ss = D::prep(sv, nchunks);
+      const T* ps_row = sv;
+      if constexpr (STAGED) {
+        ps_row = rows + (uint64_t)Cs[found] * a.ad;
+      } else {
+        stage_row(sv, base + (uint64_t)pstar * a.ad, row_bytes);
+        wave_sync();
+      }
+      const typename D::Q ss = D::prep(ps_row, nchunks);
       for (uint32_t j = idx + lane; j < M; j += kWave) {
         const uint32_t c = Ci[j];
         if (c != kPruned) {
-          const float dsp = D::dist(base + (uint64_t)c * a.ad, sv, ss, a.d, nchunks);
+          const T* r = STAGED ? rows + (uint64_t)Cs[j] * a.ad : base + (uint64_t)c * a.ad;
+          const float dsp = D::dist(r, ps_row, ss, a.d, nchunks);
           if (a.alpha * (double)dsp <= (double)Cd[j]) Ci[j] = kPruned;
         }
       }
@@ -132,6 +166,8 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
   uint32_t* sel = (uint32_t*)(my + 2u * ((a.ad * sizeof(T) + 15u) & ~15u));
   float* Cd;
   uint32_t* Ci;
+  uint16_t* Cs = (uint16_t*)(my + a.off_cand + 8u * a.ccap);  // LDS layout only (not BIG)
+  T* rows = (T*)(my + a.off_rows);
   uint32_t cap;
   if (BIG) {
     const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
@@ -170,8 +206,10 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
       }
     } else if (BIG && M64 > cap) {
       if (lane == 0) atomicOr(a.status, 2u);
+    } else if (!BIG && M64 <= a.rrows) {
+      prune_task<T, METRIC, false, true>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
     } else {
-      prune_task<T, METRIC, BIG>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci);
+      prune_task<T, METRIC, BIG, false>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
     }
     wave_sync();
   }

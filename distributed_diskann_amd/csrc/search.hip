@@ -4,6 +4,9 @@
 //
 // Per-wave LDS:  [query row][visited filter 2^bits u32][frontier A][frontier B]
 //                [kept ids][candidates][sorted candidates][slot buckets, 1 KB]
+// With GTAB the visited filter lives in global memory instead, one slice of
+// 2^bits words per resident wave (large beams: the LDS then holds only a few
+// KB per wave, and many more waves fit on a CU).
 //
 // Exactness (ids, distance bits, dist_cmps and |visited| equal to the
 // reference's):
@@ -33,14 +36,15 @@ namespace ddg {
 
 constexpr uint32_t kBuckets = 1024;  // slot buckets for the lane-conflict check
 
-template <typename T, int METRIC, int TRAV>
+template <typename T, int METRIC, int TRAV, bool GTAB>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using P = TravDist<T, METRIC, TRAV>;
   const uint32_t lane = lane_id();
   unsigned char* my = smem + (threadIdx.x >> 6) * a.lds_per_wave;
   unsigned char* ql = my;  // staged query
-  uint32_t* table = (uint32_t*)(my + a.off_table);
+  uint32_t* table = GTAB ? a.gtable + ((uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * a.gtable_words
+                        : (uint32_t*)(my + a.off_table);
   float* Fd0 = (float*)(my + a.off_f0);
   uint32_t* Fi0 = (uint32_t*)(Fd0 + a.fcap);
   float* Fd1 = (float*)(my + a.off_f1);
@@ -72,7 +76,12 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       qsrc = (const T*)a.queries + (uint64_t)q * a.ad;
     }
     stage_traversal_query<T, METRIC, TRAV>(a, ql, qsrc);
-    for (uint32_t i = lane; i <= mask; i += kWave) table[i] = 0xFFFFFFFFu;
+    if constexpr (GTAB) {
+      for (uint32_t i = lane; i < (mask + 1u) / 4u; i += kWave)
+        ((uint4*)table)[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+    } else {
+      for (uint32_t i = lane; i <= mask; i += kWave) table[i] = 0xFFFFFFFFu;
+    }
     wave_sync();
     const typename P::Q qs = P::prep(a, ql);
 
@@ -163,6 +172,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         const uint32_t av = valid ? (b0 == 0 ? nb0 : row[1 + j]) : 0u;
         const bool act = valid && ((int64_t)av != self_id);
         const uint32_t slot = (uint32_t)hash64_2((uint64_t)av) & mask;
+        if constexpr (GTAB) wave_sync_global();  // earlier table stores of other lanes
         uint32_t prev = act ? table[slot] : 0xFFFFFFFFu;
         bool has_next = false;
         // Lanes that share a slot share a bucket of `bkt`.  Every active lane
@@ -399,11 +409,17 @@ __global__ __launch_bounds__(256) void k_trav_dist(SearchArgs a, const uint32_t*
   }
 }
 
+template <typename T, int M, int TR, bool G>
+static hipError_t launch_g(const SearchArgs& a, const Launch& l, hipStream_t s) {
+  hipFuncSetAttribute((const void*)&k_search<T, M, TR, G>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                      (int)l.lds_bytes);
+  hipLaunchKernelGGL((k_search<T, M, TR, G>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
+  return hipGetLastError();
+}
+
 template <typename T, int M, int TR>
 static hipError_t launch_t(const SearchArgs& a, const Launch& l, hipStream_t s) {
-  hipFuncSetAttribute((const void*)&k_search<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes);
-  hipLaunchKernelGGL((k_search<T, M, TR>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
-  return hipGetLastError();
+  return a.gtable ? launch_g<T, M, TR, true>(a, l, s) : launch_g<T, M, TR, false>(a, l, s);
 }
 
 template <typename T, int M, int TR>
