@@ -42,7 +42,7 @@ sys.path.insert(0, HERE)
 METRIC = "search QPS at 10-recall@10=0.95 + index build sec, 1 MI355X, low-prec vs FP32"
 HBM_PEAK = 8.0e12
 BEAMS = [10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 45, 50, 55, 60, 65,
-         70, 80, 90, 100, 120, 140, 160, 180, 200]  # check_nn_recall.h:173-175, up to 200
+         70, 80, 90, 100, 120, 140, 160, 180, 200, 225, 250, 275, 300, 375, 500, 750, 1000]  # check_nn_recall.h:173-175
 ESIZE = {"native": None, "bf16": 2, "fp8": 1}
 
 
@@ -125,6 +125,19 @@ def distance_error(idx, X, Q, d, metric, mode, nq=100, m=1000, seed=17):
     return {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)), "max": float(err.max())}
 
 
+def profiled_traffic(workload, n, Ls):
+    """HBM bytes per search launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (copy-calibrated) for this exact workload and beam, from the committed
+    profile summary; None when no profile covers it."""
+    path = os.path.join(HERE, "profiles", "k_search_f32_100Kx20_Ls80.md")
+    if workload != "s" or n != 100_000 or Ls != 80 or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1]), "profiles/k_search_f32_100Kx20_Ls80.md"
+    return None, None
+
+
 def low_precision(torch, args):
     from distributed_diskann_amd import Index, datasets
     X, Q, w = datasets.generate("mips", n=args.lowp_n, nq=args.lowp_nq)
@@ -178,6 +191,9 @@ def main():
     ap.add_argument("--lowp-n", type=int, default=1_000_000, help="base points of the low-precision workload "
                                                                   "(0 skips it)")
     ap.add_argument("--lowp-nq", type=int, default=10_000)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (calibrates the FETCH_SIZE / "
+                         "WRITE_SIZE counters against known bytes) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the multi-GPU path is "
@@ -185,6 +201,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nbytes, "launches": 5}), flush=True)
+        return
 
     from distributed_diskann_amd import Index, datasets
     from distributed_diskann_amd.index import recall_with_ties
@@ -287,6 +313,7 @@ def main():
                 f"({m / cpu_build_m:.0f} points/s), QPS {qs.shape[0] / cpu_search:.0f}")
             del P, Gg
 
+    traffic, traffic_src = profiled_traffic(args.workload, n, chosen)
     out = {
         "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -302,7 +329,7 @@ def main():
         "reduced_traversal": modes,
         "low_precision": lowp,
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": f"k_search<{w.dtype},{w.metric},native>", "kernel_avg_ms": kern_avg_ms,
                      "note": (f"{X.nbytes / 1e6:.1f} MB of vectors" +
                               (": the working set is cache-resident" if X.nbytes < 256e6 else ""))},

@@ -200,24 +200,52 @@ static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nsta
     }                                                                                                      \
   } while (0)
 
+// Launch tunables, read once from the environment (for measurement sweeps):
+//   DDG_GTABLE=0|1       visited filter always in LDS / always global
+//   DDG_PRUNE_LDS=<KB>   per-wave LDS budget of robustPrune (0: no row cache)
+//   DDG_PRUNE_CCAP=<n>   candidates a prune keeps in LDS; longer lists take
+//                        the global-scratch pass
+static int env_int(const char* name, int dflt, int lo, int hi) {
+  const char* e = getenv(name);
+  if (!e) return dflt;
+  const int v = atoi(e);
+  return (v < lo || v > hi) ? dflt : v;
+}
+// Defaults from a 10M x 128 uint8 build sweep: 12 KB per wave; forward
+// prunes keep up to 512 candidates in LDS (visited lists), reverse prunes
+// 128 (|S| + R for most targets; hub targets with thousands of new
+// in-edges take the global-scratch pass instead of inflating every wave).
+static uint32_t prune_wave_lds() {
+  static const int kb = env_int("DDG_PRUNE_LDS", 12, 0, 160);
+  return (uint32_t)kb * 1024u;
+}
+static uint32_t prune_ccap_max(uint32_t dflt) {
+  static const int c = env_int("DDG_PRUNE_CCAP", 0, 64, 4096);
+  return c ? (uint32_t)c : dflt;
+}
+constexpr uint32_t kForwardCcap = 512, kReverseCcap = 128, kAnyCcap = 4096;
+
 // Per-wave LDS of a prune launch: p's and p*'s rows, the selection, ccap
 // candidates (distance, id, row slot), then a cache of rrows candidate rows
-// sized to keep the wave within kPruneWaveLds (no cache when fewer than 16
-// rows fit).
-constexpr uint32_t kPruneWaveLds = 24u * 1024u;
-static void prune_layout(const ddg_index* idx, uint32_t Rsel, uint32_t want_ccap, PruneArgs& a, Launch& l) {
+// sized to keep the wave within the prune LDS budget (no cache when fewer
+// than 16 rows fit).  Cached rows are an odd number of 16-byte chunks apart.
+static void prune_layout(const ddg_index* idx, uint32_t Rsel, uint32_t want_ccap, uint32_t ccap_max, PruneArgs& a,
+                         Launch& l) {
   const uint32_t row = round_up(idx->ad * idx->es, 16);
+  const uint32_t rs = (row / 16) % 2 == 0 ? row + 16 : row;
+  const uint32_t budget = prune_wave_lds();
+  a.rstride = rs / idx->es;
   uint32_t ccap = 64;
-  while (ccap < want_ccap && ccap < 4096) ccap <<= 1;
+  while (ccap < want_ccap && ccap < prune_ccap_max(ccap_max)) ccap <<= 1;
   auto fixed = [&](uint32_t c) { return 2 * row + round_up(Rsel * 4, 16) + round_up(c * 10, 16); };
   while (ccap > 64 && fixed(ccap) * 2 > kLdsPerCU) ccap >>= 1;
   a.ccap = ccap;
   a.off_cand = 2 * row + round_up(Rsel * 4, 16);
   a.off_rows = fixed(ccap);
-  uint32_t rrows = fixed(ccap) < kPruneWaveLds ? (kPruneWaveLds - fixed(ccap)) / row : 0u;
+  uint32_t rrows = fixed(ccap) < budget ? (budget - fixed(ccap)) / rs : 0u;
   rrows = std::min<uint32_t>(rrows, std::min<uint32_t>(ccap, round_up(std::max<uint32_t>(want_ccap, 16), 16)));
   a.rrows = rrows >= 16 ? rrows : 0u;
-  a.lds_per_wave = round_up(fixed(ccap) + a.rrows * row, 16);
+  a.lds_per_wave = round_up(fixed(ccap) + a.rrows * rs, 16);
   // the block shape (1, 2 or 4 waves) that fits the most waves on a CU
   uint32_t wpb = 1, bpc = 1, best = 0;
   for (uint32_t w : {4u, 2u, 1u}) {
@@ -314,10 +342,11 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
 }
 
 // robustPrune launch over np tasks; candidates either fixed-stride lists or CSR.
-static ddg_status run_prune(ddg_index* idx, PruneArgs a, uint64_t np, uint32_t max_cand, double* acc) {
+static ddg_status run_prune(ddg_index* idx, PruneArgs a, uint64_t np, uint32_t max_cand, double* acc,
+                            uint32_t ccap_max = kAnyCcap) {
   if (np == 0) return DDG_OK;
   Launch l{};
-  prune_layout(idx, a.Rsel, max_cand, a, l);
+  prune_layout(idx, a.Rsel, max_cand, ccap_max, a, l);
   ENSURE(B_FLAGS, 64);
   uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);  // [2] work, [3] nbig, [4] status, [5] work2
   HIPCHK(launch_fill_u32(flags + 2, 0u, 4, idx->stream));
@@ -334,11 +363,15 @@ static ddg_status run_prune(ddg_index* idx, PruneArgs a, uint64_t np, uint32_t m
   a.R1 = idx->R + 1;
   a.big_pass = 0;
   l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (np + l.waves_per_block - 1) / l.waves_per_block);
-  Timer t(idx, acc);
+  double main_ms = 0.0;
+  Timer t(idx, &main_ms);
   HIPCHK(launch_prune(idx->dtype, idx->metric, a, l, idx->stream));
   t.stop();
+  *acc += main_ms;
   uint32_t st[2];
   HIPCHK(hipMemcpy(st, flags + 3, 8, hipMemcpyDeviceToHost));
+  TRACE("  prune: %llu tasks, ccap %u, rrows %u, %u B/wave, %u waves x %u blocks: %.2f ms; %u long lists",
+        (unsigned long long)np, a.ccap, a.rrows, a.lds_per_wave, l.waves_per_block, l.blocks, main_ms, st[0]);
   if (st[0]) {
     // candidate lists beyond the LDS capacity: global scratch, one slot per wave
     uint32_t gcap = 64;
@@ -358,9 +391,12 @@ static ddg_status run_prune(ddg_index* idx, PruneArgs a, uint64_t np, uint32_t m
     lb.lds_bytes = 4 * round_up(2 * round_up(idx->ad * idx->es, 16) + round_up(a.Rsel * 4, 16), 16);
     a.lds_per_wave = lb.lds_bytes / 4;
     a.rrows = 0;
-    Timer t2(idx, acc);
+    double big_ms = 0.0;
+    Timer t2(idx, &big_ms);
     HIPCHK(launch_prune(idx->dtype, idx->metric, a, lb, idx->stream));
     t2.stop();
+    *acc += big_ms;
+    TRACE("  long-list pass (capacity %u): %.2f ms", gcap, big_ms);
   }
   HIPCHK(hipMemcpy(st, flags + 4, 4, hipMemcpyDeviceToHost));
   if (st[0] & 1u) return fail(idx, DDG_EDEGREE, "adjacency list would exceed max degree");
@@ -438,7 +474,7 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
     pa.alpha = alpha;
     pa.Rsel = Rsel;
     pa.add = 1;
-    st = run_prune(idx, pa, Bn, vcap + idx->R, &idx->prof[1]);
+    st = run_prune(idx, pa, Bn, vcap + idx->R, &idx->prof[1], kForwardCcap);
     if (st != DDG_OK) return st;
     TRACE("  forward prune done");
     // graph update + pairs (index.h:287-301)
@@ -501,7 +537,7 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
     ra.add = 1;
     ra.write_graph = 1;
     ra.append_if_fits = 1;
-    st = run_prune(idx, ra, ngroups, maxg + idx->R, &idx->prof[3]);
+    st = run_prune(idx, ra, ngroups, maxg + idx->R, &idx->prof[3], kReverseCcap);
     if (st != DDG_OK) return st;
     TRACE("  reverse edges done");
   }

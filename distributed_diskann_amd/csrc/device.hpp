@@ -112,6 +112,15 @@ __device__ __forceinline__ uint32_t upper_bound_di(const float* d, const uint32_
 }
 
 // ------------------------------------------------------------------ distances
+// A distance loads its row in batches of kChunkBatch 16-byte chunks, all in
+// flight before the arithmetic (one memory round trip per batch).
+constexpr uint32_t kChunkBatch = 8;
+
+__device__ __forceinline__ void load_chunks(uint4 (&xs)[kChunkBatch], const uint4* r, uint32_t c0, uint32_t n) {
+#pragma unroll
+  for (uint32_t u = 0; u < kChunkBatch; u++) xs[u] = c0 + u < n ? r[c0 + u] : make_uint4(0u, 0u, 0u, 0u);
+}
+
 // One lane computes one distance between a row in HBM and a reference vector
 // staged in LDS.  Rows are aligned_d elements long, a multiple of 64 bytes,
 // with zero padding (the library zeroes the padding of every device copy).
@@ -143,18 +152,24 @@ struct Dist<uint8_t, METRIC> {
     const uint4* r = (const uint4*)row;
     const uint4* v = (const uint4*)q;
     uint32_t pp = 0, qp = 0;
-    for (uint32_t c = 0; c < nchunks; c++) {
-      uint4 x = r[c];
-      uint4 y = v[c];
-      qp = __builtin_amdgcn_udot4(x.x, y.x, qp, false);
-      qp = __builtin_amdgcn_udot4(x.y, y.y, qp, false);
-      qp = __builtin_amdgcn_udot4(x.z, y.z, qp, false);
-      qp = __builtin_amdgcn_udot4(x.w, y.w, qp, false);
-      if (METRIC == L2) {
-        pp = __builtin_amdgcn_udot4(x.x, x.x, pp, false);
-        pp = __builtin_amdgcn_udot4(x.y, x.y, pp, false);
-        pp = __builtin_amdgcn_udot4(x.z, x.z, pp, false);
-        pp = __builtin_amdgcn_udot4(x.w, x.w, pp, false);
+    for (uint32_t c0 = 0; c0 < nchunks; c0 += kChunkBatch) {
+      uint4 xs[kChunkBatch];
+      load_chunks(xs, r, c0, nchunks);
+#pragma unroll
+      for (uint32_t u = 0; u < kChunkBatch; u++) {
+        if (c0 + u >= nchunks) break;
+        const uint4 x = xs[u];
+        const uint4 y = v[c0 + u];
+        qp = __builtin_amdgcn_udot4(x.x, y.x, qp, false);
+        qp = __builtin_amdgcn_udot4(x.y, y.y, qp, false);
+        qp = __builtin_amdgcn_udot4(x.z, y.z, qp, false);
+        qp = __builtin_amdgcn_udot4(x.w, y.w, qp, false);
+        if (METRIC == L2) {
+          pp = __builtin_amdgcn_udot4(x.x, x.x, pp, false);
+          pp = __builtin_amdgcn_udot4(x.y, x.y, pp, false);
+          pp = __builtin_amdgcn_udot4(x.z, x.z, pp, false);
+          pp = __builtin_amdgcn_udot4(x.w, x.w, pp, false);
+        }
       }
     }
     if (METRIC == L2) return (float)(int32_t)(st.qq + pp - 2u * qp);
@@ -182,18 +197,24 @@ struct Dist<int8_t, METRIC> {
     const uint4* r = (const uint4*)row;
     const uint4* v = (const uint4*)q;
     int32_t pp = 0, qp = 0;
-    for (uint32_t c = 0; c < nchunks; c++) {
-      uint4 x = r[c];
-      uint4 y = v[c];
-      qp = __builtin_amdgcn_sdot4((int)x.x, (int)y.x, qp, false);
-      qp = __builtin_amdgcn_sdot4((int)x.y, (int)y.y, qp, false);
-      qp = __builtin_amdgcn_sdot4((int)x.z, (int)y.z, qp, false);
-      qp = __builtin_amdgcn_sdot4((int)x.w, (int)y.w, qp, false);
-      if (METRIC == L2) {
-        pp = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, pp, false);
-        pp = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, pp, false);
-        pp = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, pp, false);
-        pp = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, pp, false);
+    for (uint32_t c0 = 0; c0 < nchunks; c0 += kChunkBatch) {
+      uint4 xs[kChunkBatch];
+      load_chunks(xs, r, c0, nchunks);
+#pragma unroll
+      for (uint32_t u = 0; u < kChunkBatch; u++) {
+        if (c0 + u >= nchunks) break;
+        const uint4 x = xs[u];
+        const uint4 y = v[c0 + u];
+        qp = __builtin_amdgcn_sdot4((int)x.x, (int)y.x, qp, false);
+        qp = __builtin_amdgcn_sdot4((int)x.y, (int)y.y, qp, false);
+        qp = __builtin_amdgcn_sdot4((int)x.z, (int)y.z, qp, false);
+        qp = __builtin_amdgcn_sdot4((int)x.w, (int)y.w, qp, false);
+        if (METRIC == L2) {
+          pp = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, pp, false);
+          pp = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, pp, false);
+          pp = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, pp, false);
+          pp = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, pp, false);
+        }
       }
     }
     if (METRIC == L2) return (float)(st.qq + pp - 2 * qp);
@@ -210,37 +231,43 @@ struct Dist<float, METRIC> {
   struct Q { int unused; };
   __device__ static Q prep(const float*, uint32_t) { return Q{0}; }
   __device__ static float dist(const float* row, const float* q, const Q&, uint32_t d, uint32_t /*nchunks*/) {
-    const float4* r4 = (const float4*)row;
+    const uint4* r4 = (const uint4*)row;
     const float4* q4 = (const float4*)q;
     float acc = 0.f;
-    uint32_t i = 0;
-    if (METRIC == L2) {
-      for (; i + 4 <= d; i += 4) {
-        float4 a = r4[i >> 2];
-        float4 b = q4[i >> 2];
-        float e0 = a.x - b.x, e1 = a.y - b.y, e2 = a.z - b.z, e3 = a.w - b.w;
-        float g = e0 * e0;
-        g = g + e1 * e1;
-        g = g + e2 * e2;
-        g = g + e3 * e3;
-        acc = acc + g;
+    const uint32_t ng = d >> 2;  // whole groups of four, in order, then the tail
+    for (uint32_t g0 = 0; g0 < ng; g0 += kChunkBatch) {
+      uint4 xs[kChunkBatch];
+      load_chunks(xs, r4, g0, ng);
+#pragma unroll
+      for (uint32_t u = 0; u < kChunkBatch; u++) {
+        if (g0 + u >= ng) break;
+        const float4 a = make_float4(__uint_as_float(xs[u].x), __uint_as_float(xs[u].y), __uint_as_float(xs[u].z),
+                                     __uint_as_float(xs[u].w));
+        const float4 b = q4[g0 + u];
+        if (METRIC == L2) {
+          float e0 = a.x - b.x, e1 = a.y - b.y, e2 = a.z - b.z, e3 = a.w - b.w;
+          float g = e0 * e0;
+          g = g + e1 * e1;
+          g = g + e2 * e2;
+          g = g + e3 * e3;
+          acc = acc + g;
+        } else {
+          acc = acc + b.x * a.x;
+          acc = acc + b.y * a.y;
+          acc = acc + b.z * a.z;
+          acc = acc + b.w * a.w;
+        }
       }
-      for (; i < d; i++) {
+    }
+    for (uint32_t i = ng << 2; i < d; i++) {
+      if (METRIC == L2) {
         float e = row[i] - q[i];
         acc = acc + e * e;
+      } else {
+        acc = acc + q[i] * row[i];
       }
-      return acc;
     }
-    for (; i + 4 <= d; i += 4) {
-      float4 a = r4[i >> 2];
-      float4 b = q4[i >> 2];
-      acc = acc + b.x * a.x;
-      acc = acc + b.y * a.y;
-      acc = acc + b.z * a.z;
-      acc = acc + b.w * a.w;
-    }
-    for (; i < d; i++) acc = acc + q[i] * row[i];
-    return -acc;
+    return METRIC == L2 ? acc : -acc;
   }
 };
 

@@ -66,8 +66,9 @@ struct PruneArgs {
   double alpha;
   uint32_t a_stride, d, ad, R1, Rsel, ccap, add, write_graph, append_if_fits;
   uint32_t lds_per_wave, off_cand, gcap, big_pass;
-  // LDS row cache: tasks with at most rrows candidates stage their rows at off_rows
-  uint32_t rrows, off_rows;
+  // LDS row cache: tasks with at most rrows candidates stage their rows at
+  // off_rows, rstride elements apart
+  uint32_t rrows, off_rows, rstride;
 };
 
 struct Launch {

@@ -19,7 +19,9 @@
 //
 // When a task's candidate rows fit the per-wave LDS row cache (rrows rows),
 // they are staged there once (STAGED): d(p, c) and every d(p*, p') of the
-// greedy sweep then read LDS instead of HBM, with the same arithmetic.
+// greedy sweep then read LDS instead of HBM, with the same arithmetic.  The
+// cache stride is an odd number of 16-byte chunks, so the lanes of a wave
+// reading the same chunk of different rows spread over all LDS banks.
 #include "device.hpp"
 #include "internal.hpp"
 
@@ -61,7 +63,7 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
           const uint32_t e = e0 + u * kWave;
           if (e < total) {
             const uint32_t j = e / nchunks, c = e - j * nchunks;
-            ((uint4*)(rows + (uint64_t)j * a.ad))[c] = v[u];
+            ((uint4*)(rows + (uint64_t)j * a.rstride))[c] = v[u];
           }
         }
       }
@@ -70,7 +72,7 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
     const typename D::Q ps = D::prep(pv, nchunks);
     for (uint32_t j = lane; j < M; j += kWave) {
       const uint32_t id = Ci[j];
-      const T* r = STAGED ? rows + (uint64_t)j * a.ad : base + (uint64_t)id * a.ad;
+      const T* r = STAGED ? rows + (uint64_t)j * a.rstride : base + (uint64_t)id * a.ad;
       Cd[j] = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(r, pv, ps, a.d, nchunks);
       if (STAGED) Cs[j] = (uint16_t)j;
     }
@@ -126,7 +128,7 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
       if (nsel >= a.Rsel) break;  // the reference's remaining prunes cannot change the result
       const T* ps_row = sv;
       if constexpr (STAGED) {
-        ps_row = rows + (uint64_t)Cs[found] * a.ad;
+        ps_row = rows + (uint64_t)Cs[found] * a.rstride;
       } else {
         stage_row(sv, base + (uint64_t)pstar * a.ad, row_bytes);
         wave_sync();
@@ -135,7 +137,7 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
       for (uint32_t j = idx + lane; j < M; j += kWave) {
         const uint32_t c = Ci[j];
         if (c != kPruned) {
-          const T* r = STAGED ? rows + (uint64_t)Cs[j] * a.ad : base + (uint64_t)c * a.ad;
+          const T* r = STAGED ? rows + (uint64_t)Cs[j] * a.rstride : base + (uint64_t)c * a.ad;
           const float dsp = D::dist(r, ps_row, ss, a.d, nchunks);
           if (a.alpha * (double)dsp <= (double)Cd[j]) Ci[j] = kPruned;
         }

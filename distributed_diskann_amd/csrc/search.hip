@@ -116,7 +116,13 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     int pos = fsz > 0 ? 0 : -1;
     // Adjacency row of the vertex predicted to be expanded next, loaded while
     // the current expansion is merged (degree and the first 64 neighbors).
-    uint32_t pf_id = 0xFFFFFFFFu, pf_deg = 0, pf_nb = 0;
+    // The prediction (the smallest unvisited entry of frontier and new
+    // candidates) is the vertex the next expansion takes whenever the search
+    // continues.  With a global filter the table words of those neighbors are
+    // loaded ahead too: no table store happens between that load and the
+    // next expansion's filter, so they are the words the filter would read.
+    uint32_t pf_id = 0xFFFFFFFFu, pf_deg = 0, pf_nb = 0, pf_prev = 0xFFFFFFFFu;
+    bool pf_table = false;
 
     while (pos >= 0 && nvis < a.limit) {
       float* FD = cur ? Fd1 : Fd0;
@@ -125,6 +131,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       const uint32_t cid = FI[pos] & kIdMask;
       const uint32_t* row = a.graph + (uint64_t)cid * R1;
       uint32_t deg, nb0;
+      const bool use_pf_table = GTAB && pf_table && pf_id == cid;
       if (pf_id == cid) {
         deg = pf_deg;
         nb0 = pf_nb;
@@ -173,7 +180,8 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         const bool act = valid && ((int64_t)av != self_id);
         const uint32_t slot = (uint32_t)hash64_2((uint64_t)av) & mask;
         if constexpr (GTAB) wave_sync_global();  // earlier table stores of other lanes
-        uint32_t prev = act ? table[slot] : 0xFFFFFFFFu;
+        uint32_t prev = 0xFFFFFFFFu;
+        if (act) prev = (b0 == 0 && use_pf_table) ? pf_prev : table[slot];
         bool has_next = false;
         // Lanes that share a slot share a bucket of `bkt`.  Every active lane
         // writes its lane id into its bucket; a lane that reads back another
@@ -312,6 +320,14 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         }
         wave_sync();
         cur ^= 1u;
+      }
+      if constexpr (GTAB) {
+        pf_table = pf_id != 0xFFFFFFFFu;
+        if (pf_table) {
+          wave_sync_global();  // this expansion's table stores by other lanes
+          const uint32_t ps = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
+          pf_prev = lane < a.R ? table[ps] : 0xFFFFFFFFu;
+        }
       }
       float* FD2 = cur ? Fd1 : Fd0;
       uint32_t* FI2 = cur ? Fi1 : Fi0;

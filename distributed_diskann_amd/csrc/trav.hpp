@@ -61,18 +61,25 @@ struct TravDist<float, METRIC, TRAV_BF16> {
     const uint4* r = (const uint4*)((const unsigned char*)a.trav_rows + (uint64_t)id * a.trav_stride);
     const float4* q = (const float4*)ql;
     float s0 = 0.f, s1 = 0.f;
-    for (uint32_t c = 0; c < a.trav_chunks; c++) {
-      const uint4 x = r[c];
-      const float4 qa = q[2 * c];
-      const float4 qb = q[2 * c + 1];
-      acc_term<METRIC>(s0, qa.x, bf16_lo(x.x));
-      acc_term<METRIC>(s1, qa.y, bf16_hi(x.x));
-      acc_term<METRIC>(s0, qa.z, bf16_lo(x.y));
-      acc_term<METRIC>(s1, qa.w, bf16_hi(x.y));
-      acc_term<METRIC>(s0, qb.x, bf16_lo(x.z));
-      acc_term<METRIC>(s1, qb.y, bf16_hi(x.z));
-      acc_term<METRIC>(s0, qb.z, bf16_lo(x.w));
-      acc_term<METRIC>(s1, qb.w, bf16_hi(x.w));
+    for (uint32_t c0 = 0; c0 < a.trav_chunks; c0 += kChunkBatch) {
+      uint4 xs[kChunkBatch];
+      load_chunks(xs, r, c0, a.trav_chunks);
+#pragma unroll
+      for (uint32_t u = 0; u < kChunkBatch; u++) {
+        if (c0 + u >= a.trav_chunks) break;
+        const uint32_t c = c0 + u;
+        const uint4 x = xs[u];
+        const float4 qa = q[2 * c];
+        const float4 qb = q[2 * c + 1];
+        acc_term<METRIC>(s0, qa.x, bf16_lo(x.x));
+        acc_term<METRIC>(s1, qa.y, bf16_hi(x.x));
+        acc_term<METRIC>(s0, qa.z, bf16_lo(x.y));
+        acc_term<METRIC>(s1, qa.w, bf16_hi(x.y));
+        acc_term<METRIC>(s0, qb.x, bf16_lo(x.z));
+        acc_term<METRIC>(s1, qb.y, bf16_hi(x.z));
+        acc_term<METRIC>(s0, qb.z, bf16_lo(x.w));
+        acc_term<METRIC>(s1, qb.w, bf16_hi(x.w));
+      }
     }
     const float s = s0 + s1;
     return METRIC == L2 ? s : -s;
@@ -110,12 +117,19 @@ struct TravDist<float, METRIC, TRAV_FP8> {
     const float4* q = (const float4*)ql;
     const float4* sc = (const float4*)(ql + a.off_qaux);
     float s0 = 0.f, s1 = 0.f;
-    for (uint32_t c = 0; c < a.trav_chunks; c++) {
-      const uint4 x = r[c];
-      fp8_word<METRIC>(s0, s1, x.x, q[4 * c + 0], sc[4 * c + 0]);
-      fp8_word<METRIC>(s0, s1, x.y, q[4 * c + 1], sc[4 * c + 1]);
-      fp8_word<METRIC>(s0, s1, x.z, q[4 * c + 2], sc[4 * c + 2]);
-      fp8_word<METRIC>(s0, s1, x.w, q[4 * c + 3], sc[4 * c + 3]);
+    for (uint32_t c0 = 0; c0 < a.trav_chunks; c0 += kChunkBatch) {
+      uint4 xs[kChunkBatch];
+      load_chunks(xs, r, c0, a.trav_chunks);
+#pragma unroll
+      for (uint32_t u = 0; u < kChunkBatch; u++) {
+        if (c0 + u >= a.trav_chunks) break;
+        const uint32_t c = c0 + u;
+        const uint4 x = xs[u];
+        fp8_word<METRIC>(s0, s1, x.x, q[4 * c + 0], sc[4 * c + 0]);
+        fp8_word<METRIC>(s0, s1, x.y, q[4 * c + 1], sc[4 * c + 1]);
+        fp8_word<METRIC>(s0, s1, x.z, q[4 * c + 2], sc[4 * c + 2]);
+        fp8_word<METRIC>(s0, s1, x.w, q[4 * c + 3], sc[4 * c + 3]);
+      }
     }
     const float s = s0 + s1;
     return METRIC == L2 ? s : -s;
