@@ -847,7 +847,7 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   ddg_status pst = prepare_trav(idx, trav, nullptr);
   if (pst != DDG_OK) return pst;
   if (!search_layout(idx, beam_max, nstarts, trav, a, l))
-    return fail(idx, DDG_EINVAL, "beam width too large for the LDS-resident visited filter (max 256)");
+    return fail(idx, DDG_EINVAL, "beam width too large for the LDS frontier");
   ENSURE(B_FLAGS, 64);
   uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
   HIPCHK(launch_fill_u32(flags, 0u, 2, idx->stream));

@@ -40,6 +40,21 @@ def test_search_reference_n0(n0, Ls):
     assert np.array_equal(vis, z[f"visited_{Ls}"])
 
 
+@pytest.mark.parametrize("Ls", [300, 1000])
+def test_search_large_beams(oracle, xs, Ls):
+    """Beams of the reference's sweep beyond 256: the visited filter (up to
+    2^18 words) lives in global memory."""
+    print(f"[gpu] search large beam Ls={Ls}", flush=True)
+    idx = _index(xs["X"], xs["d"], xs["G"])
+    Q = xs["Q"][:200]
+    ids, dists, cmps, vis = idx.search(Q, 10, Ls, 1.35)
+    P = oracle.Points(xs["X"], xs["d"])
+    oi, od, oc, ov = oracle.search_all(P, oracle.Graph(xs["G"]), Q, 10, Ls, cut=1.35)
+    assert np.array_equal(ids, oi)
+    assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+    assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+
+
 def test_search_per_query_beam(oracle, xs):
     print("[gpu] per-query beam", flush=True)
     rng = np.random.default_rng(7)
