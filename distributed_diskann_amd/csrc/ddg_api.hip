@@ -31,7 +31,18 @@ struct Scratch {
 enum BufId {
   B_QUERIES, B_STARTS, B_QBEAM, B_QIDS, B_OUT_IDS, B_OUT_D, B_CMPS, B_VISN, B_VIS_IDS, B_VIS_D, B_FLAGS,
   B_INSERTS, B_NEW_IDS, B_NEW_LEN, B_OFFS, B_KEYS, B_VALS, B_KEYS2, B_VALS2, B_TMP, B_GFLAGS, B_GOFFS, B_GKEYS,
-  B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D, B_GTAB
+  B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D, B_GTAB, B_BIGOFF, B_RPTS
+};
+
+// Device words of B_FLAGS (u32 index): [0] search work counter, [1] visited
+// list overflow, [4] error status (bit 0 degree overflow, bit 1 prune scratch
+// exhausted), [8..9] pair count (u64), [10..11] group count (u64), [12]
+// largest group, [16] prune work counter, [17] long-list count, [18]
+// long-list work counter, [20..21] long-list scratch handed out (u64).
+constexpr size_t kFlagBytes = 256;
+enum FlagWord : uint32_t {
+  F_WORK = 0, F_OVERFLOW = 1, F_STATUS = 4, F_PAIRS = 8, F_GROUPS = 10, F_MAXG = 12, F_PWORK = 16, F_NBIG = 17,
+  F_PWORK2 = 18, F_GUSED = 20
 };
 
 }  // namespace
@@ -259,11 +270,18 @@ static void prune_layout(const ddg_index* idx, uint32_t Rsel, uint32_t want_ccap
 }
 
 // ---------------------------------------------------------------- timing
+// Device time between construction and stop() (HIP events on the handle's
+// stream; stop() synchronises).  A disabled timer records nothing, so it can
+// sit in code that a stream capture records.
 struct Timer {
   ddg_index* idx;
   double* acc;
-  Timer(ddg_index* i, double* a) : idx(i), acc(a) { (void)hipEventRecord(idx->ev0, idx->stream); }
+  bool on;
+  Timer(ddg_index* i, double* a, bool enabled = true) : idx(i), acc(a), on(enabled) {
+    if (on) (void)hipEventRecord(idx->ev0, idx->stream);
+  }
   void stop() {
+    if (!on) return;
     (void)hipEventRecord(idx->ev1, idx->stream);
     (void)hipEventSynchronize(idx->ev1);
     float ms = 0;
@@ -288,11 +306,11 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
     ENSURE(B_VIS_D, (size_t)B * *vcap * 4);
     ENSURE(B_VISN, (size_t)B * 4);
     ENSURE(B_CMPS, (size_t)B * 8);
-    ENSURE(B_FLAGS, 64);
+    ENSURE(B_FLAGS, kFlagBytes);
     ENSURE(B_STARTS, 64);
     HIPCHK(hipMemcpyAsync(buf<uint32_t>(idx, B_STARTS), &start, 4, hipMemcpyHostToDevice, idx->stream));
-    uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);  // [0] work, [1] overflow
-    HIPCHK(launch_fill_u32(flags, 0u, 16, idx->stream));
+    uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+    HIPCHK(launch_fill_u32(flags + F_WORK, 0u, 2, idx->stream));
     SearchArgs a{};
     Launch l{};
     if (!search_layout(idx, L, 1, DDG_TRAV_NATIVE, a, l))
@@ -341,67 +359,116 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
   return fail(idx, DDG_EINVAL, "visited lists exceed the capacity limit");
 }
 
-// robustPrune launch over np tasks; candidates either fixed-stride lists or CSR.
-static ddg_status run_prune(ddg_index* idx, PruneArgs a, uint64_t np, uint32_t max_cand, double* acc,
-                            uint32_t ccap_max = kAnyCcap) {
-  if (np == 0) return DDG_OK;
-  Launch l{};
-  prune_layout(idx, a.Rsel, max_cand, ccap_max, a, l);
-  ENSURE(B_FLAGS, 64);
-  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);  // [2] work, [3] nbig, [4] status, [5] work2
-  HIPCHK(launch_fill_u32(flags + 2, 0u, 4, idx->stream));
-  ENSURE(B_BIG, (size_t)np * 4);
+// Long-list scratch a prune can need: a task takes the global path only when
+// its M = |A| + deg candidates exceed ccap, and takes pow2(M) < 2M entries.
+// With `total` candidates over np tasks and deg <= Radd, the sum is below
+// 2 (total + np Radd), and below 2 total (1 + Radd / (ccap - Radd)) because
+// each such task has |A| > ccap - Radd.
+static uint64_t prune_pool(uint64_t total, uint64_t np, uint32_t Radd, uint32_t ccap, uint32_t max_cand) {
+  if (max_cand <= ccap) return 0;
+  uint64_t b = 2 * (total + np * (uint64_t)Radd);
+  if (ccap > Radd) b = std::min<uint64_t>(b, 2 * total + (2 * total * Radd + (ccap - Radd) - 1) / (ccap - Radd));
+  return b + 64;
+}
+
+// Layout, launch shapes and buffers of a prune over at most np tasks whose
+// lists hold at most max_cand candidates (total in all).  Allocates, so it
+// runs before any recorded (captured) launch.
+struct PrunePlan {
+  PruneArgs a;
+  Launch l, lb;
+};
+// max_exact: max_cand bounds every list (per-wave scratch slots); otherwise
+// max_cand only sizes the layout and long lists take bump-allocated slices.
+static ddg_status prune_prepare(ddg_index* idx, PrunePlan& pp, uint64_t np, uint32_t max_cand, uint64_t total,
+                                uint32_t ccap_max, bool max_exact) {
+  PruneArgs& a = pp.a;
+  prune_layout(idx, a.Rsel, max_cand, ccap_max, a, pp.l);
+  const uint32_t big_waves = 4 * idx->num_cu;
+  uint64_t pool = 0;
+  a.gcap = 0;
+  bool bump = false;
+  if (max_cand > a.ccap) {
+    if (max_exact) {
+      uint64_t g = 64;
+      while (g < max_cand) g <<= 1;
+      a.gcap = (uint32_t)g;
+      pool = g * big_waves;
+    } else {
+      bump = true;
+      pool = prune_pool(total, np, a.add ? idx->R : 0u, a.ccap, max_cand);
+    }
+  }
+  ENSURE(B_FLAGS, kFlagBytes);
+  ENSURE(B_BIG, std::max<uint64_t>(np, 1) * 4);
+  ENSURE(B_BIGOFF, std::max<uint64_t>(np, 1) * 8);
+  ENSURE(B_GSD, std::max<uint64_t>(pool, 64) * 4);
+  ENSURE(B_GSI, std::max<uint64_t>(pool, 64) * 4);
+  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
   a.base = idx->rows;
   a.graph = idx->graph;
-  a.work = flags + 2;
-  a.nbig = flags + 3;
-  a.status = flags + 4;
+  a.work = flags + F_PWORK;
+  a.nbig = flags + F_NBIG;
+  a.status = flags + F_STATUS;
+  a.gused = (uint64_t*)(flags + F_GUSED);
   a.big_tasks = buf<uint32_t>(idx, B_BIG);
+  a.big_off = bump ? buf<uint64_t>(idx, B_BIGOFF) : nullptr;
+  a.gscratch_d = buf<float>(idx, B_GSD);
+  a.gscratch_i = buf<uint32_t>(idx, B_GSI);
+  a.gpool = pool;
   a.np = np;
   a.d = idx->d;
   a.ad = idx->ad;
   a.R1 = idx->R + 1;
-  a.big_pass = 0;
-  l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (np + l.waves_per_block - 1) / l.waves_per_block);
-  double main_ms = 0.0;
-  Timer t(idx, &main_ms);
-  HIPCHK(launch_prune(idx->dtype, idx->metric, a, l, idx->stream));
-  t.stop();
-  *acc += main_ms;
-  uint32_t st[2];
-  HIPCHK(hipMemcpy(st, flags + 3, 8, hipMemcpyDeviceToHost));
-  TRACE("  prune: %llu tasks, ccap %u, rrows %u, %u B/wave, %u waves x %u blocks: %.2f ms; %u long lists",
-        (unsigned long long)np, a.ccap, a.rrows, a.lds_per_wave, l.waves_per_block, l.blocks, main_ms, st[0]);
-  if (st[0]) {
-    // candidate lists beyond the LDS capacity: global scratch, one slot per wave
-    uint32_t gcap = 64;
-    while (gcap < max_cand) gcap <<= 1;
-    const uint32_t waves = 4 * idx->num_cu;
-    ENSURE(B_GSD, (size_t)waves * gcap * 4);
-    ENSURE(B_GSI, (size_t)waves * gcap * 4);
-    a.gscratch_d = buf<float>(idx, B_GSD);
-    a.gscratch_i = buf<uint32_t>(idx, B_GSI);
-    a.gcap = gcap;
-    a.big_pass = 1;
-    a.work = flags + 5;
-    HIPCHK(launch_fill_u32(flags + 5, 0u, 1, idx->stream));
-    Launch lb = l;
-    lb.waves_per_block = 4;
-    lb.blocks = idx->num_cu;
-    lb.lds_bytes = 4 * round_up(2 * round_up(idx->ad * idx->es, 16) + round_up(a.Rsel * 4, 16), 16);
-    a.lds_per_wave = lb.lds_bytes / 4;
-    a.rrows = 0;
-    double big_ms = 0.0;
-    Timer t2(idx, &big_ms);
-    HIPCHK(launch_prune(idx->dtype, idx->metric, a, lb, idx->stream));
-    t2.stop();
-    *acc += big_ms;
-    TRACE("  long-list pass (capacity %u): %.2f ms", gcap, big_ms);
-  }
-  HIPCHK(hipMemcpy(st, flags + 4, 4, hipMemcpyDeviceToHost));
-  if (st[0] & 1u) return fail(idx, DDG_EDEGREE, "adjacency list would exceed max degree");
-  if (st[0] & 2u) return fail(idx, DDG_EINVAL, "prune candidate list exceeds scratch capacity");
+  pp.l.blocks = (uint32_t)std::min<uint64_t>(pp.l.blocks, (np + pp.l.waves_per_block - 1) / pp.l.waves_per_block);
+  pp.lb.waves_per_block = 4;
+  pp.lb.blocks = big_waves / 4;
+  pp.lb.lds_bytes = 4 * round_up(2 * round_up(idx->ad * idx->es, 16) + round_up(a.Rsel * 4, 16), 16);
   return DDG_OK;
+}
+
+// Both launches of a prepared prune, no host synchronisation.
+static hipError_t prune_enqueue(ddg_index* idx, const PrunePlan& pp) {
+  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+  hipError_t e = launch_fill_u32(flags + F_PWORK, 0u, F_GUSED + 2 - F_PWORK, idx->stream);
+  if (e != hipSuccess) return e;
+  PruneArgs a = pp.a;
+  a.big_pass = 0;
+  e = launch_prune(idx->dtype, idx->metric, a, pp.l, idx->stream);
+  if (e != hipSuccess) return e;
+  a.big_pass = 1;
+  a.work = flags + F_PWORK2;
+  a.lds_per_wave = pp.lb.lds_bytes / 4;
+  a.rrows = 0;
+  return launch_prune(idx->dtype, idx->metric, a, pp.lb, idx->stream);
+}
+
+static ddg_status check_status(ddg_index* idx) {
+  uint32_t st = 0;
+  HIPCHK(hipMemcpy(&st, buf<uint32_t>(idx, B_FLAGS) + F_STATUS, 4, hipMemcpyDeviceToHost));
+  if (st & 1u) return fail(idx, DDG_EDEGREE, "adjacency list would exceed max degree");
+  if (st & 2u) return fail(idx, DDG_EINVAL, "prune candidate list exceeds scratch capacity");
+  return DDG_OK;
+}
+
+// robustPrune over np tasks (fixed-stride lists or CSR) with `total`
+// candidates in all, timed into *acc, errors checked.
+static ddg_status run_prune(ddg_index* idx, PruneArgs a, uint64_t np, uint32_t max_cand, uint64_t total, double* acc,
+                            uint32_t ccap_max = kAnyCcap) {
+  if (np == 0) return DDG_OK;
+  PrunePlan pp{};
+  pp.a = a;
+  ddg_status st = prune_prepare(idx, pp, np, max_cand, total, ccap_max, true);
+  if (st != DDG_OK) return st;
+  HIPCHK(launch_fill_u32(buf<uint32_t>(idx, B_FLAGS) + F_STATUS, 0u, 1, idx->stream));
+  double ms = 0.0;
+  Timer t(idx, &ms);
+  HIPCHK(prune_enqueue(idx, pp));
+  t.stop();
+  *acc += ms;
+  TRACE("  prune: %llu tasks, ccap %u, rrows %u, %u B/wave, %u waves x %u blocks: %.2f ms",
+        (unsigned long long)np, pp.a.ccap, pp.a.rrows, pp.a.lds_per_wave, pp.l.waves_per_block, pp.l.blocks, ms);
+  return check_status(idx);
 }
 
 static void schedule(uint64_t m, double base, size_t max_batch, std::vector<std::pair<uint64_t, uint64_t>>& out) {
@@ -441,6 +508,131 @@ __global__ void k_max_group(const uint64_t* goff, const uint64_t* ngroups, uint3
 }
 
 // knn_index::batch_insert over device ids ins[0..m)
+// Everything of a build round after its insert search (index.h:281-323): the
+// forward prune, the graph update and pair list, the radix sort and grouping
+// of the pairs, the reverse append / prune.  Every size it needs beyond B is
+// produced on the device (pair and group counts, long-list queue), so the
+// whole sequence can be recorded once as a hipGraph and replayed for every
+// round of the same B.  The round's points are in B_RPTS, its visited lists
+// in B_VIS_* with stride vcap.
+struct RoundShape {
+  uint64_t B;
+  uint32_t vcap, Rsel, kb;
+  double alpha;
+};
+
+static ddg_status round_prepare(ddg_index* idx, const RoundShape& r, PrunePlan& fwd, PrunePlan& rev) {
+  const uint64_t B = r.B, Pmax = r.B * r.Rsel;
+  ENSURE(B_NEW_IDS, (size_t)B * r.Rsel * 4);
+  ENSURE(B_NEW_LEN, (size_t)B * 4);
+  ENSURE(B_OFFS, (size_t)(B + 1) * 8);
+  ENSURE(B_KEYS, Pmax * 4);
+  ENSURE(B_VALS, Pmax * 4);
+  ENSURE(B_KEYS2, Pmax * 4);
+  ENSURE(B_VALS2, Pmax * 4);
+  ENSURE(B_TMP, std::max(scan_tmp_bytes(std::max(B, Pmax)), radix_tmp_bytes(Pmax)) + 64);
+  ENSURE(B_GFLAGS, Pmax * 4);
+  ENSURE(B_GOFFS, Pmax * 8);
+  ENSURE(B_GKEYS, Pmax * 4);
+  ENSURE(B_GOFF, (Pmax + 1) * 8);
+  // forward prune (index.h:281): visited lists with their distances, plus G[p]
+  fwd.a = PruneArgs{};
+  fwd.a.pts = buf<uint32_t>(idx, B_RPTS);
+  fwd.a.a_ids = buf<uint32_t>(idx, B_VIS_IDS);
+  fwd.a.a_dists = buf<float>(idx, B_VIS_D);
+  fwd.a.a_len = buf<uint32_t>(idx, B_VISN);
+  fwd.a.a_stride = r.vcap;
+  fwd.a.out_ids = buf<uint32_t>(idx, B_NEW_IDS);
+  fwd.a.out_len = buf<uint32_t>(idx, B_NEW_LEN);
+  fwd.a.alpha = r.alpha;
+  fwd.a.Rsel = r.Rsel;
+  fwd.a.add = 1;
+  ddg_status st = prune_prepare(idx, fwd, B, r.vcap + idx->R, B * (uint64_t)r.vcap, kForwardCcap, true);
+  if (st != DDG_OK) return st;
+  // reverse append / prune per target (index.h:309-322); the cap keeps at
+  // least 2R so the long-list pool bound holds
+  rev.a = PruneArgs{};
+  rev.a.pts = buf<uint32_t>(idx, B_GKEYS);
+  rev.a.a_off = buf<uint64_t>(idx, B_GOFF);
+  rev.a.alpha = r.alpha;
+  rev.a.Rsel = r.Rsel;
+  rev.a.add = 1;
+  rev.a.write_graph = 1;
+  rev.a.append_if_fits = 1;
+  uint32_t rcap = kReverseCcap;
+  while (rcap < 2 * idx->R) rcap <<= 1;
+  return prune_prepare(idx, rev, Pmax, rcap + 1 + idx->R, Pmax, rcap, false);
+}
+
+// Enqueues the round; no host synchronisation unless `timed` (then each part
+// is timed into the profile, which synchronises).
+static ddg_status round_enqueue(ddg_index* idx, const RoundShape& r, const PrunePlan& fwd0, const PrunePlan& rev0,
+                                bool timed) {
+  const uint64_t B = r.B, Pmax = r.B * r.Rsel;
+  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+  uint64_t* dpairs = (uint64_t*)(flags + F_PAIRS);
+  uint64_t* dgroups = (uint64_t*)(flags + F_GROUPS);
+  double t_fwd = 0.0, t_edges = 0.0, t_rev = 0.0;
+  {
+    Timer t(idx, &t_fwd, timed);
+    HIPCHK(prune_enqueue(idx, fwd0));
+    t.stop();
+  }
+  const bool in_alt = Pmax > 1 && ((r.kb + 7) / 8) % 2 == 1;
+  uint32_t* skeys = in_alt ? buf<uint32_t>(idx, B_KEYS2) : buf<uint32_t>(idx, B_KEYS);
+  uint32_t* svals = in_alt ? buf<uint32_t>(idx, B_VALS2) : buf<uint32_t>(idx, B_VALS);
+  {
+    Timer t(idx, &t_edges, timed);
+    HIPCHK(launch_apply_rows(idx->graph, idx->R + 1, buf<uint32_t>(idx, B_RPTS), buf<uint32_t>(idx, B_NEW_IDS),
+                             buf<uint32_t>(idx, B_NEW_LEN), r.Rsel, B, flags + F_STATUS, idx->stream));
+    HIPCHK(scan_exclusive_u32(buf<uint32_t>(idx, B_NEW_LEN), buf<uint64_t>(idx, B_OFFS), B, dpairs,
+                              buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap, idx->stream));
+    HIPCHK(launch_emit_pairs(buf<uint32_t>(idx, B_RPTS), buf<uint32_t>(idx, B_NEW_IDS), buf<uint32_t>(idx, B_NEW_LEN),
+                             buf<uint64_t>(idx, B_OFFS), r.Rsel, B, buf<uint32_t>(idx, B_KEYS),
+                             buf<uint32_t>(idx, B_VALS), idx->stream));
+    bool alt = false;
+    HIPCHK(radix_sort_pairs(buf<uint32_t>(idx, B_KEYS), buf<uint32_t>(idx, B_VALS), buf<uint32_t>(idx, B_KEYS2),
+                            buf<uint32_t>(idx, B_VALS2), Pmax, r.kb, buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap,
+                            idx->stream, &alt, dpairs));
+    if (Pmax > 1 && alt != in_alt) return fail(idx, DDG_EDEVICE, "radix pass count mismatch");
+    HIPCHK(launch_group_starts(skeys, Pmax, buf<uint32_t>(idx, B_GFLAGS), buf<uint64_t>(idx, B_GOFFS), dgroups,
+                               buf<uint32_t>(idx, B_GKEYS), buf<uint64_t>(idx, B_GOFF), buf<void>(idx, B_TMP),
+                               idx->s.b[B_TMP].cap, idx->stream, dpairs));
+    t.stop();
+  }
+  {
+    PrunePlan rev = rev0;
+    rev.a.a_ids = svals;
+    rev.a.np_dev = dgroups;
+    Timer t(idx, &t_rev, timed);
+    HIPCHK(prune_enqueue(idx, rev));
+    t.stop();
+  }
+  if (timed) {
+    idx->prof[1] += t_fwd;
+    idx->prof[2] += t_edges;
+    idx->prof[3] += t_rev;
+  }
+  return DDG_OK;
+}
+
+// Round tails run as recorded hipGraphs (one per batch size B), unless
+// DDG_GRAPH=0 (then they run launch by launch with per-part timing).
+static bool use_graphs() {
+  static const int g = env_int("DDG_GRAPH", 1, 0, 1);
+  return g == 1;
+}
+
+struct GraphCache {
+  std::vector<std::pair<uint64_t, hipGraphExec_t>> execs;
+  uint32_t vcap = 0;
+  ~GraphCache() { clear(); }
+  void clear() {
+    for (auto& e : execs) (void)hipGraphExecDestroy(e.second);
+    execs.clear();
+  }
+};
+
 static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t m, double alpha, uint32_t L,
                                    uint32_t Rsel, uint32_t start, double base, double max_fraction,
                                    uint64_t max_batch_base, uint32_t* vc_dev) {
@@ -448,99 +640,75 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
   schedule(m, base, max_batch_size(max_fraction, max_batch_base), batches);
   uint32_t vcap = std::max<uint32_t>(2 * L + 64, 128);
   const uint32_t kb = key_bits_for(idx->n);
-  uint64_t total_cmps = 0;
+  uint64_t total_cmps = 0, Bmax = 0;
+  for (auto [lo, hi] : batches) Bmax = std::max<uint64_t>(Bmax, hi - lo);
+  if (Bmax == 0) return DDG_OK;
+  // every buffer at its largest size up front: recorded rounds keep their pointers
+  ENSURE(B_FLAGS, kFlagBytes);
+  ENSURE(B_RPTS, Bmax * 4);
+  ENSURE(B_VIS_IDS, (size_t)Bmax * vcap * 4);
+  ENSURE(B_VIS_D, (size_t)Bmax * vcap * 4);
+  ENSURE(B_VISN, (size_t)Bmax * 4);
+  ENSURE(B_CMPS, (size_t)Bmax * 8);
+  {
+    PrunePlan f{}, r{};
+    ddg_status st = round_prepare(idx, RoundShape{Bmax, vcap, Rsel, kb, alpha}, f, r);
+    if (st != DDG_OK) return st;
+  }
+  HIPCHK(launch_fill_u32(buf<uint32_t>(idx, B_FLAGS) + F_STATUS, 0u, 1, idx->stream));
+  const bool graphs = use_graphs() && !trace_on();
+  GraphCache cache;
   for (auto [lo, hi] : batches) {
     const uint64_t Bn = hi - lo;
     if (Bn == 0) continue;
     const uint32_t* pts = ins + lo;
     TRACE("batch [%llu, %llu): search", (unsigned long long)lo, (unsigned long long)hi);
+    const uint32_t vcap_before = vcap;
     ddg_status st = run_insert_search(idx, pts, Bn, L, start, &vcap, &total_cmps);
     if (st != DDG_OK) return st;
     TRACE("  search done (vcap %u)", vcap);
+    if (vcap != vcap_before) cache.clear();  // visited buffers were reallocated
+    st = check_status(idx);  // the previous round's errors (the search synchronised)
+    if (st != DDG_OK) return st;
     if (vc_dev)
       hipLaunchKernelGGL(k_acc_visited, dim3((Bn + 255) / 256), dim3(256), 0, idx->stream, vc_dev, pts,
                          buf<uint32_t>(idx, B_VISN), Bn);
-    // forward prune (index.h:281)
-    ENSURE(B_NEW_IDS, (size_t)Bn * Rsel * 4);
-    ENSURE(B_NEW_LEN, (size_t)Bn * 4);
-    PruneArgs pa{};
-    pa.pts = pts;
-    pa.a_ids = buf<uint32_t>(idx, B_VIS_IDS);
-    pa.a_dists = buf<float>(idx, B_VIS_D);
-    pa.a_len = buf<uint32_t>(idx, B_VISN);
-    pa.a_stride = vcap;
-    pa.out_ids = buf<uint32_t>(idx, B_NEW_IDS);
-    pa.out_len = buf<uint32_t>(idx, B_NEW_LEN);
-    pa.alpha = alpha;
-    pa.Rsel = Rsel;
-    pa.add = 1;
-    st = run_prune(idx, pa, Bn, vcap + idx->R, &idx->prof[1], kForwardCcap);
+    HIPCHK(hipMemcpyAsync(buf<uint32_t>(idx, B_RPTS), pts, Bn * 4, hipMemcpyDeviceToDevice, idx->stream));
+    const RoundShape rs{Bn, vcap, Rsel, kb, alpha};
+    PrunePlan fwd{}, rev{};
+    st = round_prepare(idx, rs, fwd, rev);
     if (st != DDG_OK) return st;
-    TRACE("  forward prune done");
-    // graph update + pairs (index.h:287-301)
-    Timer te(idx, &idx->prof[2]);
-    ENSURE(B_OFFS, (size_t)(Bn + 1) * 8);
-    ENSURE(B_TMP, std::max(scan_tmp_bytes(Bn), radix_tmp_bytes((uint64_t)Bn * Rsel)) + 64);
-    ENSURE(B_FLAGS, 64);
-    uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
-    HIPCHK(launch_fill_u32(flags + 4, 0u, 1, idx->stream));
-    HIPCHK(launch_apply_rows(idx->graph, idx->R + 1, pts, buf<uint32_t>(idx, B_NEW_IDS), buf<uint32_t>(idx, B_NEW_LEN),
-                             Rsel, Bn, flags + 4, idx->stream));
-    uint64_t* dtotal = (uint64_t*)(flags + 8);
-    HIPCHK(scan_exclusive_u32(buf<uint32_t>(idx, B_NEW_LEN), buf<uint64_t>(idx, B_OFFS), Bn, dtotal,
-                              buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap, idx->stream));
-    uint64_t P = 0;
-    HIPCHK(hipMemcpyAsync(&P, dtotal, 8, hipMemcpyDeviceToHost, idx->stream));
-    HIPCHK(hipStreamSynchronize(idx->stream));
-    TRACE("  applied rows, %llu pairs", (unsigned long long)P);
-    if (P == 0) { te.stop(); continue; }
-    ENSURE(B_KEYS, P * 4);
-    ENSURE(B_VALS, P * 4);
-    ENSURE(B_KEYS2, P * 4);
-    ENSURE(B_VALS2, P * 4);
-    ENSURE(B_TMP, std::max(scan_tmp_bytes(P), radix_tmp_bytes(P)) + 64);
-    HIPCHK(launch_emit_pairs(pts, buf<uint32_t>(idx, B_NEW_IDS), buf<uint32_t>(idx, B_NEW_LEN), buf<uint64_t>(idx, B_OFFS),
-                             Rsel, Bn, buf<uint32_t>(idx, B_KEYS), buf<uint32_t>(idx, B_VALS), idx->stream));
-    bool in_alt = false;
-    HIPCHK(radix_sort_pairs(buf<uint32_t>(idx, B_KEYS), buf<uint32_t>(idx, B_VALS), buf<uint32_t>(idx, B_KEYS2),
-                            buf<uint32_t>(idx, B_VALS2), P, kb, buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap, idx->stream,
-                            &in_alt));
-    uint32_t* skeys = in_alt ? buf<uint32_t>(idx, B_KEYS2) : buf<uint32_t>(idx, B_KEYS);
-    uint32_t* svals = in_alt ? buf<uint32_t>(idx, B_VALS2) : buf<uint32_t>(idx, B_VALS);
-    ENSURE(B_GFLAGS, P * 4);
-    ENSURE(B_GOFFS, P * 8);
-    ENSURE(B_GKEYS, P * 4);
-    ENSURE(B_GOFF, (P + 1) * 8);
-    uint64_t* dng = (uint64_t*)(flags + 10);
-    HIPCHK(launch_group_starts(skeys, P, buf<uint32_t>(idx, B_GFLAGS), buf<uint64_t>(idx, B_GOFFS), dng,
-                               buf<uint32_t>(idx, B_GKEYS), buf<uint64_t>(idx, B_GOFF), buf<void>(idx, B_TMP),
-                               idx->s.b[B_TMP].cap, idx->stream));
-    HIPCHK(launch_fill_u32(flags + 12, 0u, 1, idx->stream));
-    hipLaunchKernelGGL(k_max_group, dim3((P + 255) / 256), dim3(256), 0, idx->stream, buf<uint64_t>(idx, B_GOFF), dng,
-                       flags + 12);
-    uint32_t hostf[9];
-    HIPCHK(hipMemcpyAsync(hostf, flags + 4, 9 * 4, hipMemcpyDeviceToHost, idx->stream));
-    HIPCHK(hipStreamSynchronize(idx->stream));
-    te.stop();
-    if (hostf[0] & 1u) return fail(idx, DDG_EDEGREE, "update_neighbors: cannot exceed max degree");
-    uint64_t ngroups = 0;
-    std::memcpy(&ngroups, &hostf[6], 8);
-    const uint32_t maxg = hostf[8];
-    TRACE("  sorted pairs: %llu groups, largest %u", (unsigned long long)ngroups, maxg);
-    // reverse edges: append or prune per target (index.h:309-322)
-    PruneArgs ra{};
-    ra.pts = buf<uint32_t>(idx, B_GKEYS);
-    ra.a_ids = svals;
-    ra.a_off = buf<uint64_t>(idx, B_GOFF);
-    ra.alpha = alpha;
-    ra.Rsel = Rsel;
-    ra.add = 1;
-    ra.write_graph = 1;
-    ra.append_if_fits = 1;
-    st = run_prune(idx, ra, ngroups, maxg + idx->R, &idx->prof[3], kReverseCcap);
-    if (st != DDG_OK) return st;
-    TRACE("  reverse edges done");
+    if (!graphs) {
+      st = round_enqueue(idx, rs, fwd, rev, true);
+      if (st != DDG_OK) return st;
+      TRACE("  round tail done");
+      continue;
+    }
+    hipGraphExec_t exec = nullptr;
+    for (auto& e : cache.execs)
+      if (e.first == Bn) exec = e.second;
+    if (!exec) {
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(idx->stream, hipStreamCaptureModeThreadLocal));
+      st = round_enqueue(idx, rs, fwd, rev, false);
+      hipError_t ce = hipStreamEndCapture(idx->stream, &g);
+      if (st != DDG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      if (ce != hipSuccess) return fail(idx, DDG_EDEVICE, std::string("round capture: ") + hipGetErrorString(ce));
+      ce = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ce != hipSuccess) return fail(idx, DDG_EDEVICE, std::string("round instantiate: ") + hipGetErrorString(ce));
+      cache.execs.push_back({Bn, exec});
+    }
+    Timer t(idx, &idx->prof[2]);
+    HIPCHK(hipGraphLaunch(exec, idx->stream));
+    t.stop();
   }
+  HIPCHK(hipStreamSynchronize(idx->stream));
+  ddg_status st = check_status(idx);
+  if (st != DDG_OK) return st;
   idx->prof[7] += (double)total_cmps;
   return DDG_OK;
 }
@@ -636,6 +804,12 @@ ddg_status ddg_create(ddg_dtype dtype, ddg_metric metric, uint64_t n, uint32_t d
   idx->es = es;
   int dev = 0;
   hipGetDevice(&dev);
+  static bool attrs_set = false;  // per process: kernel attributes are process-wide
+  if (!attrs_set) {
+    search_set_attributes((int)kLdsPerCU);
+    prune_set_attributes((int)kLdsPerCU);
+    attrs_set = true;
+  }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     idx->num_cu = (uint32_t)prop.multiProcessorCount;
@@ -829,7 +1003,7 @@ ddg_status ddg_robust_prune(ddg_index* idx, uint64_t np, const uint32_t* pts, co
   a.Rsel = R;
   a.add = add ? 1 : 0;
   for (double& p : idx->prof) p = 0;
-  ddg_status st = run_prune(idx, a, np, maxc + (add ? idx->R : 0), &idx->prof[1]);
+  ddg_status st = run_prune(idx, a, np, maxc + (add ? idx->R : 0), nc, &idx->prof[1]);
   if (st != DDG_OK) return st;
   HIPCHK(hipMemcpy(out_ids, buf<uint32_t>(idx, B_NEW_IDS), np * R * 4, hipMemcpyDeviceToHost));
   HIPCHK(hipMemcpy(out_len, buf<uint32_t>(idx, B_NEW_LEN), np * 4, hipMemcpyDeviceToHost));
@@ -848,7 +1022,7 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   if (pst != DDG_OK) return pst;
   if (!search_layout(idx, beam_max, nstarts, trav, a, l))
     return fail(idx, DDG_EINVAL, "beam width too large for the LDS frontier");
-  ENSURE(B_FLAGS, 64);
+  ENSURE(B_FLAGS, kFlagBytes);
   uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
   HIPCHK(launch_fill_u32(flags, 0u, 2, idx->stream));
   a.base = idx->rows;
@@ -1070,7 +1244,10 @@ ddg_status ddg_internal_prune(ddg_index* idx, uint64_t np, const uint32_t* pts, 
   a.add = add ? 1 : 0;
   a.write_graph = write_graph ? 1 : 0;
   a.append_if_fits = append_if_fits ? 1 : 0;
-  ddg_status st = run_prune(idx, a, np, max_cand, &idx->prof[1]);
+  uint64_t total = 0;
+  if (a_off) total = a_off[np] - a_off[0];
+  else for (uint64_t t = 0; t < np; t++) total += a_len[t];
+  ddg_status st = run_prune(idx, a, np, max_cand, total, &idx->prof[1]);
   if (st != DDG_OK) return st;
   if (!write_graph) {
     HIPCHK(hipMemcpy(out_ids, buf<void>(idx, B_NEW_IDS), np * Rsel * 4, hipMemcpyDeviceToHost));
@@ -1097,7 +1274,7 @@ ddg_status ddg_internal_group_pairs(ddg_index* idx, const uint32_t* keys, const 
   ENSURE(B_GOFFS, n * 8);
   ENSURE(B_GKEYS, n * 4);
   ENSURE(B_GOFF, (n + 1) * 8);
-  ENSURE(B_FLAGS, 64);
+  ENSURE(B_FLAGS, kFlagBytes);
   uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
   HIPCHK(hipMemcpy(buf<void>(idx, B_KEYS), keys, n * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(buf<void>(idx, B_VALS), vals, n * 4, hipMemcpyHostToDevice));

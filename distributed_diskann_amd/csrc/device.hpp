@@ -16,6 +16,7 @@ constexpr uint32_t kWave = 64;
 constexpr uint32_t kFlag = 0x80000000u;   // frontier entry already visited
 constexpr uint32_t kIdMask = 0x7FFFFFFFu;
 constexpr uint32_t kPruned = 0xFFFFFFFFu; // index.h:102 sentinel (-1)
+constexpr uint32_t kNoTask = 0xFFFFFFFFu; // queue slot without a task
 
 enum Dtype : int { U8 = 0, I8 = 1, F32 = 2 };
 enum Metric : int { L2 = 0, MIPS = 1 };

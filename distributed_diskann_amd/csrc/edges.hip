@@ -94,8 +94,18 @@ __device__ __forceinline__ uint64_t block_scan_excl(uint64_t v, uint64_t* sh, ui
   return woff + x - v;
 }
 
-__global__ void k_scan_reduce(const uint32_t* in, uint64_t n, uint64_t* bsum) {
+// Sizes given as (n, n_dev): n bounds the grid, n_dev (nullable) holds the
+// count the kernel honours, so a launch can be recorded before its input
+// size is known (graph capture of a build round).
+__device__ __forceinline__ uint64_t dev_count(uint64_t n, const uint64_t* n_dev) {
+  if (!n_dev) return n;
+  const uint64_t v = *n_dev;
+  return v < n ? v : n;
+}
+
+__global__ void k_scan_reduce(const uint32_t* in, uint64_t n_max, const uint64_t* n_dev, uint64_t* bsum) {
   __shared__ uint64_t sh[4];
+  const uint64_t n = dev_count(n_max, n_dev);
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
   uint64_t s = 0;
   for (uint32_t k = 0; k < kItems; k++) {
@@ -122,8 +132,10 @@ __global__ void k_scan_sums(uint64_t* bsum, uint64_t nb, uint64_t* total) {
   if (threadIdx.x == 0 && total) *total = carry;
 }
 
-__global__ void k_scan_down(const uint32_t* in, uint64_t n, const uint64_t* bsum, uint64_t* out) {
+__global__ void k_scan_down(const uint32_t* in, uint64_t n_max, const uint64_t* n_dev, const uint64_t* bsum,
+                            uint64_t* out) {
   __shared__ uint64_t sh[4];
+  const uint64_t n = dev_count(n_max, n_dev);
   const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)threadIdx.x * kItems;
   uint32_t v[kItems];
   uint64_t s = 0;
@@ -144,14 +156,14 @@ __global__ void k_scan_down(const uint32_t* in, uint64_t n, const uint64_t* bsum
 size_t scan_tmp_bytes(uint64_t n) { return ((n + kTile - 1) / kTile + 1) * sizeof(uint64_t); }
 
 hipError_t scan_exclusive_u32(const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total, void* tmp,
-                              size_t tmp_bytes, hipStream_t s) {
+                              size_t tmp_bytes, hipStream_t s, const uint64_t* n_dev) {
   if (n == 0) return hipSuccess;
   const uint32_t nb = blocks_for(n, kTile);
   if (tmp_bytes < scan_tmp_bytes(n)) return hipErrorInvalidValue;
   uint64_t* bsum = (uint64_t*)tmp;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, bsum);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n, n_dev, bsum);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, bsum, (uint64_t)nb, total);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, in, n, bsum, out);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kBlock), 0, s, in, n, n_dev, bsum, out);
   return hipGetLastError();
 }
 
@@ -198,9 +210,10 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t dig, bool valid) {
   return m;
 }
 
-__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint64_t n, uint32_t shift,
-                                                    uint32_t* hist, uint32_t nblocks) {
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint64_t n_max, const uint64_t* n_dev,
+                                                    uint32_t shift, uint32_t* hist, uint32_t nblocks) {
   __shared__ uint32_t h[kRadix];
+  const uint64_t n = dev_count(n_max, n_dev);
   h[threadIdx.x] = 0;
   __syncthreads();
   const uint64_t base = (uint64_t)blockIdx.x * kTile;
@@ -212,10 +225,11 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* keys, uint64
   hist[(uint64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, const uint32_t* vals, uint64_t n,
-                                                       uint32_t shift, const uint64_t* hoff, uint32_t nblocks,
-                                                       uint32_t* okeys, uint32_t* ovals) {
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* keys, const uint32_t* vals, uint64_t n_max,
+                                                       const uint64_t* n_dev, uint32_t shift, const uint64_t* hoff,
+                                                       uint32_t nblocks, uint32_t* okeys, uint32_t* ovals) {
   __shared__ uint32_t cnt[4][kRadix];
+  const uint64_t n = dev_count(n_max, n_dev);
   __shared__ uint64_t wbase[4][kRadix];
   const uint32_t lane = lane_id();
   const uint32_t w = threadIdx.x >> 6;
@@ -269,7 +283,8 @@ size_t radix_tmp_bytes(uint64_t n) {
 }
 
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint64_t n,
-                            uint32_t key_bits, void* tmp, size_t tmp_bytes, hipStream_t s, bool* result_in_alt) {
+                            uint32_t key_bits, void* tmp, size_t tmp_bytes, hipStream_t s, bool* result_in_alt,
+                            const uint64_t* n_dev) {
   *result_in_alt = false;
   if (n <= 1) return hipSuccess;
   if (tmp_bytes < radix_tmp_bytes(n)) return hipErrorInvalidValue;
@@ -282,10 +297,10 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
   uint32_t *ik = keys, *iv = vals, *ok = keys_alt, *ov = vals_alt;
   bool alt = false;
   for (uint32_t shift = 0; shift < key_bits; shift += 8) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, s, ik, n, shift, hist, nb);
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kBlock), 0, s, ik, n, n_dev, shift, hist, nb);
     hipError_t e = scan_exclusive_u32(hist, hoff, h, nullptr, stmp, stmp_bytes, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, s, ik, iv, n, shift, hoff, nb, ok, ov);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kBlock), 0, s, ik, iv, n, n_dev, shift, hoff, nb, ok, ov);
     uint32_t* t;
     t = ik; ik = ok; ok = t;
     t = iv; iv = ov; ov = t;
@@ -296,13 +311,16 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, 
 }
 
 // ------------------------------------------------------------- group starts
-__global__ void k_group_flags(const uint32_t* keys, uint64_t n, uint32_t* flags) {
+__global__ void k_group_flags(const uint32_t* keys, uint64_t n_max, const uint64_t* n_dev, uint32_t* flags) {
+  const uint64_t n = dev_count(n_max, n_dev);
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
 }
 
-__global__ void k_group_write(const uint32_t* keys, const uint32_t* flags, const uint64_t* offs, uint64_t n,
-                              uint32_t* group_keys, uint64_t* group_off, const uint64_t* ngroups) {
+__global__ void k_group_write(const uint32_t* keys, const uint32_t* flags, const uint64_t* offs, uint64_t n_max,
+                              const uint64_t* n_dev, uint32_t* group_keys, uint64_t* group_off,
+                              const uint64_t* ngroups) {
+  const uint64_t n = dev_count(n_max, n_dev);
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && flags[i]) {
     group_keys[offs[i]] = keys[i];
@@ -313,13 +331,13 @@ __global__ void k_group_write(const uint32_t* keys, const uint32_t* flags, const
 
 hipError_t launch_group_starts(const uint32_t* keys, uint64_t n, uint32_t* flags, uint64_t* offs,
                                uint64_t* ngroups_dev, uint32_t* group_keys, uint64_t* group_off, void* tmp,
-                               size_t tmp_bytes, hipStream_t s) {
+                               size_t tmp_bytes, hipStream_t s, const uint64_t* n_dev) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_group_flags, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, flags);
-  hipError_t e = scan_exclusive_u32(flags, offs, n, ngroups_dev, tmp, tmp_bytes, s);
+  hipLaunchKernelGGL(k_group_flags, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, n, n_dev, flags);
+  hipError_t e = scan_exclusive_u32(flags, offs, n, ngroups_dev, tmp, tmp_bytes, s, n_dev);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_group_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, flags, offs, n, group_keys,
-                     group_off, ngroups_dev);
+  hipLaunchKernelGGL(k_group_write, dim3(blocks_for(n, 256)), dim3(256), 0, s, keys, flags, offs, n, n_dev,
+                     group_keys, group_off, ngroups_dev);
   return hipGetLastError();
 }
 

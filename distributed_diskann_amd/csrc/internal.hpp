@@ -60,8 +60,12 @@ struct PruneArgs {
   uint32_t* big_tasks;         // tasks whose candidate list exceeds the LDS capacity
   uint32_t* nbig;
   uint32_t* status;            // [0]: degree overflow
-  float* gscratch_d;           // global candidate arrays for big tasks (per wave)
+  float* gscratch_d;           // global candidate arrays of long-list tasks (bump-allocated)
   uint32_t* gscratch_i;
+  uint64_t* big_off;           // scratch offset of each queued long-list task
+  uint64_t* gused;             // scratch entries handed out so far
+  uint64_t gpool;              // scratch entries available
+  const uint64_t* np_dev;      // nullable: task count on the device (np then bounds it)
   uint64_t np;
   double alpha;
   uint32_t a_stride, d, ad, R1, Rsel, ccap, add, write_graph, append_if_fits;
@@ -89,22 +93,29 @@ hipError_t launch_make_bf16(const float* rows, uint64_t n, uint32_t ad, uint32_t
 hipError_t launch_make_fp8(const float* rows, uint64_t n, uint32_t ad, uint32_t d, const float* inv_scale, void* out,
                            uint32_t stride, hipStream_t s);
 hipError_t launch_prune(int dtype, int metric, const PruneArgs& a, const Launch& l, hipStream_t s);
+// raise the dynamic-LDS limit of every search / prune kernel (once per process)
+void search_set_attributes(int max_lds);
+void prune_set_attributes(int max_lds);
 
 // edges.hip
 hipError_t launch_apply_rows(uint32_t* graph, uint32_t R1, const uint32_t* pts, const uint32_t* new_ids,
                              const uint32_t* new_len, uint32_t Rsel, uint64_t B, uint32_t* status, hipStream_t s);
+// Launchers taking (n, n_dev): n sizes the grid; when n_dev is not null the
+// kernels process min(*n_dev, n) elements (the count is produced on the
+// device by an earlier kernel of the same round).
 hipError_t scan_exclusive_u32(const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total, void* tmp,
-                              size_t tmp_bytes, hipStream_t s);
+                              size_t tmp_bytes, hipStream_t s, const uint64_t* n_dev = nullptr);
 size_t scan_tmp_bytes(uint64_t n);
 hipError_t launch_emit_pairs(const uint32_t* pts, const uint32_t* new_ids, const uint32_t* new_len,
                              const uint64_t* offs, uint32_t Rsel, uint64_t B, uint32_t* keys, uint32_t* vals,
                              hipStream_t s);
 size_t radix_tmp_bytes(uint64_t n);
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint64_t n,
-                            uint32_t key_bits, void* tmp, size_t tmp_bytes, hipStream_t s, bool* result_in_alt);
+                            uint32_t key_bits, void* tmp, size_t tmp_bytes, hipStream_t s, bool* result_in_alt,
+                            const uint64_t* n_dev = nullptr);
 hipError_t launch_group_starts(const uint32_t* keys, uint64_t n, uint32_t* flags, uint64_t* offs,
                                uint64_t* ngroups_dev, uint32_t* group_keys, uint64_t* group_off, void* tmp,
-                               size_t tmp_bytes, hipStream_t s);
+                               size_t tmp_bytes, hipStream_t s, const uint64_t* n_dev = nullptr);
 hipError_t launch_sort_rows(int dtype, int metric, const void* base, uint32_t* graph, uint64_t n, uint32_t R1,
                             uint32_t d, uint32_t ad, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);

@@ -14,8 +14,11 @@
 // (graph.h:84-93), else replace G[p] by the pruned list; index.h:309-322).
 //
 // A task whose candidate list exceeds the LDS capacity is queued in
-// big_tasks and re-run by a second launch (big_pass = 1) that keeps its
-// candidates in a per-wave global scratch instead.
+// big_tasks; a second launch (big_pass = 1) prunes the queued tasks in global
+// scratch: a slot of gcap entries per wave when the host knows the longest
+// list, otherwise a slice from a device bump allocator (big_off) over a pool
+// the host sized so it cannot run out.  Both launches read their task counts
+// on the device, so a round can be recorded before its sizes exist.
 //
 // When a task's candidate rows fit the per-wave LDS row cache (rrows rows),
 // they are staged there once (STAGED): d(p, c) and every d(p*, p') of the
@@ -166,27 +169,26 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
   T* pv = (T*)my;                                   // p's row
   T* sv = (T*)(my + ((a.ad * sizeof(T) + 15u) & ~15u));  // p*'s row
   uint32_t* sel = (uint32_t*)(my + 2u * ((a.ad * sizeof(T) + 15u) & ~15u));
-  float* Cd;
-  uint32_t* Ci;
-  uint16_t* Cs = (uint16_t*)(my + a.off_cand + 8u * a.ccap);  // LDS layout only (not BIG)
+  float* Cd = (float*)(my + a.off_cand);                       // LDS layout (not BIG)
+  uint32_t* Ci = (uint32_t*)(Cd + a.ccap);
+  uint16_t* Cs = (uint16_t*)(my + a.off_cand + 8u * a.ccap);
   T* rows = (T*)(my + a.off_rows);
-  uint32_t cap;
-  if (BIG) {
-    const uint64_t gw = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wave;
-    Cd = a.gscratch_d + gw * a.gcap;
-    Ci = a.gscratch_i + gw * a.gcap;
-    cap = a.gcap;
-  } else {
-    Cd = (float*)(my + a.off_cand);
-    Ci = (uint32_t*)(Cd + a.ccap);
-    cap = a.ccap;
-  }
-  const uint64_t ntasks = BIG ? (uint64_t)a.nbig[0] : a.np;
+  const uint32_t cap = a.ccap;
+  uint64_t ntasks = a.np;
+  if (BIG) ntasks = a.nbig[0];
+  else if (a.np_dev && a.np_dev[0] < ntasks) ntasks = a.np_dev[0];
 
   for (;;) {
     const uint32_t tq = fetch_task(a.work);
     if (tq >= ntasks) break;
     const uint32_t t = BIG ? a.big_tasks[tq] : tq;
+    if (t == kNoTask) continue;  // fetch_task reconverges the wave first
+    if (BIG) {
+      const uint64_t off = a.big_off ? a.big_off[tq]
+                                     : ((uint64_t)blockIdx.x * (blockDim.x >> 6) + wave) * a.gcap;
+      Cd = a.gscratch_d + off;
+      Ci = a.gscratch_i + off;
+    }
     const uint32_t p = a.pts[t];
     uint32_t* grow = a.graph + (uint64_t)p * a.R1;
     uint64_t a0, alen;
@@ -203,11 +205,25 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
       if (lane == 0) grow[0] = deg + (uint32_t)alen;
     } else if (!BIG && M64 > cap) {
       if (lane == 0) {
-        uint32_t s = atomicAdd(a.nbig, 1u);
-        a.big_tasks[s] = t;
+        const uint32_t s = atomicAdd(a.nbig, 1u);
+        if (a.big_off) {
+          uint64_t need = 1;
+          while (need < M64) need <<= 1;  // the bitonic sort pads to a power of two
+          const uint64_t off = atomicAdd((unsigned long long*)a.gused, (unsigned long long)need);
+          if (off + need <= a.gpool) {
+            a.big_tasks[s] = t;
+            a.big_off[s] = off;
+          } else {
+            a.big_tasks[s] = kNoTask;
+            atomicOr(a.status, 2u);
+          }
+        } else if (M64 <= a.gcap) {
+          a.big_tasks[s] = t;
+        } else {
+          a.big_tasks[s] = kNoTask;
+          atomicOr(a.status, 2u);
+        }
       }
-    } else if (BIG && M64 > cap) {
-      if (lane == 0) atomicOr(a.status, 2u);
     } else if (!BIG && M64 <= a.rrows) {
       prune_task<T, METRIC, false, true>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
     } else {
@@ -218,9 +234,24 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
 }
 
 template <typename T, int M>
+static void set_attrs_t(int lds) {
+  (void)hipFuncSetAttribute((const void*)&k_prune<T, M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void*)&k_prune<T, M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
+void prune_set_attributes(int max_lds) {
+  set_attrs_t<uint8_t, L2>(max_lds);
+  set_attrs_t<uint8_t, MIPS>(max_lds);
+  set_attrs_t<int8_t, L2>(max_lds);
+  set_attrs_t<int8_t, MIPS>(max_lds);
+  set_attrs_t<float, L2>(max_lds);
+  set_attrs_t<float, MIPS>(max_lds);
+}
+
+// (the dynamic-LDS limit is raised once by prune_set_attributes, outside any
+// stream capture)
+template <typename T, int M>
 static hipError_t launch_t(const PruneArgs& a, const Launch& l, hipStream_t s) {
-  hipFuncSetAttribute((const void*)&k_prune<T, M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes);
-  hipFuncSetAttribute((const void*)&k_prune<T, M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds_bytes);
   if (a.big_pass)
     hipLaunchKernelGGL((k_prune<T, M, true>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
   else

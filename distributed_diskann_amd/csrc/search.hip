@@ -427,8 +427,6 @@ __global__ __launch_bounds__(256) void k_trav_dist(SearchArgs a, const uint32_t*
 
 template <typename T, int M, int TR, bool G>
 static hipError_t launch_g(const SearchArgs& a, const Launch& l, hipStream_t s) {
-  hipFuncSetAttribute((const void*)&k_search<T, M, TR, G>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)l.lds_bytes);
   hipLaunchKernelGGL((k_search<T, M, TR, G>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
   return hipGetLastError();
 }
@@ -441,11 +439,31 @@ static hipError_t launch_t(const SearchArgs& a, const Launch& l, hipStream_t s) 
 template <typename T, int M, int TR>
 static hipError_t launch_d(const SearchArgs& a, const uint32_t* ids, uint32_t m, float* out, const Launch& l,
                            hipStream_t s) {
-  hipFuncSetAttribute((const void*)&k_trav_dist<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                      (int)l.lds_bytes);
   hipLaunchKernelGGL((k_trav_dist<T, M, TR>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a,
                      ids, m, out);
   return hipGetLastError();
+}
+
+template <typename T, int M, int TR>
+static void set_attrs_t(int lds) {
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void*)&k_trav_dist<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+}
+
+// Raises the dynamic-LDS limit of every search kernel once (outside any
+// stream capture); the launchers then only launch.
+void search_set_attributes(int max_lds) {
+  set_attrs_t<uint8_t, L2, TRAV_NATIVE>(max_lds);
+  set_attrs_t<uint8_t, MIPS, TRAV_NATIVE>(max_lds);
+  set_attrs_t<int8_t, L2, TRAV_NATIVE>(max_lds);
+  set_attrs_t<int8_t, MIPS, TRAV_NATIVE>(max_lds);
+  set_attrs_t<float, L2, TRAV_NATIVE>(max_lds);
+  set_attrs_t<float, MIPS, TRAV_NATIVE>(max_lds);
+  set_attrs_t<float, L2, TRAV_BF16>(max_lds);
+  set_attrs_t<float, MIPS, TRAV_BF16>(max_lds);
+  set_attrs_t<float, L2, TRAV_FP8>(max_lds);
+  set_attrs_t<float, MIPS, TRAV_FP8>(max_lds);
 }
 
 hipError_t launch_search(int dtype, int metric, int trav, const SearchArgs& a, const Launch& l, hipStream_t s) {
