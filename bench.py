@@ -158,6 +158,7 @@ def low_precision(torch, args):
         rerank = 0 if mode == "native" else 100
         prep = idx.prepare_traversal(mode)
         Ls, rec = sweep(ds, gi, gd, mode, rerank)
+        idx.autotune_search(Q, 10, Ls, trav=mode, rerank=rerank)
         wall, kern = ds.timed(Ls, max(3, args.steps // 4), 1, mode, rerank)
         cm, vs = ds.counters()
         s_trav = 4 if mode == "native" else ESIZE[mode]
@@ -236,6 +237,8 @@ def main():
     ds = DeviceSearch(torch, idx, Q)
     chosen, rec0 = sweep(ds, gt_ids, gt_d)
     log(f"[bench] sweep -> Ls={chosen} (recall {rec0:.4f})")
+    tune = idx.autotune_search(Q, 10, chosen)
+    log(f"[bench] autotune at Ls={chosen}: {tune}")
     ms_per_step, kern_avg_ms = ds.timed(chosen, args.steps, args.warmup)
     qps = nq / (ms_per_step / 1000.0)
     rec = recall_with_ties(ds.result_ids(), gt_ids, gt_d, 10)
@@ -248,6 +251,7 @@ def main():
     # the per-query-beam launch of the workload
     beams = datasets.s_query_beams(nq)
     pqb = torch.from_numpy(beams.astype(np.int32)).to(torch.device("cuda:0"))
+    idx.autotune_search(Q, 10, int(beams.max()))  # the launch shape of the largest beam class
     mixed_ms, mixed_kern = ds.timed(chosen, max(1, args.steps // 4), 1, pqb=pqb, bmax=int(beams.max()))
     mixed_rec = recall_with_ties(ds.result_ids(), gt_ids, gt_d, 10)
     log(f"[bench] per-query beams: {mixed_ms:.3f} ms/step, recall {mixed_rec:.4f}")
@@ -259,6 +263,7 @@ def main():
         for mode in ("bf16", "fp8"):
             prep = idx.prepare_traversal(mode)
             Ls_m, r_m = sweep(ds, gt_ids, gt_d, mode, 100)
+            idx.autotune_search(Q, 10, Ls_m, trav=mode, rerank=100)
             wall_m, kern_m = ds.timed(Ls_m, max(3, args.steps // 4), 1, mode, 100)
             cm_m, vs_m = ds.counters()
             B_m = alg_bytes(cm_m, vs_m, nq, w.d, ESIZE[mode], avg_deg, min(100, Ls_m))
@@ -322,6 +327,7 @@ def main():
         "config": {"workload": f"{n}x{w.d} {w.dtype} {w.metric}, R={w.R} L={w.L} alpha={w.alpha}, k=10, Ls={chosen}",
                    "n": n, "d": w.d, "nq": nq, "R": w.R, "L": w.L, "alpha": w.alpha, "k": 10, "Ls": chosen},
         "recall": rec,
+        "autotune": tune,
         "build_seconds": build_s,
         "build_profile_ms": {k: v for k, v in bprof.items() if k.endswith("_ms")},
         "per_query_beam": {"qps": nq / (mixed_ms / 1000.0), "kernel_ms": mixed_kern, "recall": mixed_rec,

@@ -59,6 +59,14 @@ struct ddg_index {
   void* trav_rows[3] = {nullptr, nullptr, nullptr};
   uint32_t trav_stride[3] = {0, 0, 0};
   float* trav_scale = nullptr;  // FP8: s_j at [0, d), 1/s_j at [d, 2d)
+  // search launch shapes picked by the autotuner, per (traversal, filter bits)
+  struct Tuned {
+    uint32_t key;
+    int gtab;
+    uint32_t wpb, bpc;
+  };
+  std::vector<Tuned> tuned;
+  const Tuned* cfg_override = nullptr;  // set while the autotuner times a shape
   uint32_t num_cu = 256;
   Scratch s;
   std::string err;
@@ -169,13 +177,23 @@ static bool global_table(uint32_t bits) {
   return (4u << bits) > kLdsTableMax;
 }
 
+static uint32_t tune_key(int trav, uint32_t bits) { return ((uint32_t)trav << 8) | bits; }
+
 // Fills the LDS layout and launch shape of a search; a.gtable_words > 0 asks
 // the caller for a global filter pool of blocks * waves_per_block slices.
+// The filter placement and block shape come from, in order: the autotuner's
+// shape being timed, a tuned shape for (trav, filter bits), the defaults
+// (filter in LDS up to 8 KB, up to 4 waves per block, as many blocks per CU
+// as LDS and 32 waves allow).  Returns false when the shape does not fit.
 static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, int trav, SearchArgs& a,
-                          Launch& l) {
+                          Launch& l, bool use_tuned = true) {
   const uint32_t qrow = trav_layout(idx, trav, a);
   const uint32_t bits = filter_bits(beam_max);
-  const bool gtab = global_table(bits);
+  const ddg_index::Tuned* cfg = idx->cfg_override;
+  if (!cfg && use_tuned)
+    for (const auto& t : idx->tuned)
+      if (t.key == tune_key(trav, bits)) cfg = &t;
+  const bool gtab = cfg ? cfg->gtab != 0 : global_table(bits);
   const uint32_t table = gtab ? 0u : 4u << bits;
   a.gtable = nullptr;
   a.gtable_words = gtab ? 1u << bits : 0u;
@@ -195,8 +213,14 @@ static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nsta
   a.rcap = rcap;
   if (a.lds_per_wave > kLdsPerCU) return false;
   uint32_t wpb = std::min<uint32_t>(4, kLdsPerCU / a.lds_per_wave);
+  if (cfg) wpb = cfg->wpb;
   const uint32_t per_block = wpb * a.lds_per_wave;
+  if (per_block > kLdsPerCU) return false;
   uint32_t bpc = std::max<uint32_t>(1, std::min<uint32_t>(32 / wpb, kLdsPerCU / per_block));
+  if (cfg) {
+    if (cfg->bpc * per_block > kLdsPerCU || cfg->bpc * wpb > 32) return false;
+    bpc = cfg->bpc;
+  }
   l.waves_per_block = wpb;
   l.lds_bytes = per_block;
   l.blocks = idx->num_cu * bpc;
@@ -313,7 +337,7 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
     HIPCHK(launch_fill_u32(flags + F_WORK, 0u, 2, idx->stream));
     SearchArgs a{};
     Launch l{};
-    if (!search_layout(idx, L, 1, DDG_TRAV_NATIVE, a, l))
+    if (!search_layout(idx, L, 1, DDG_TRAV_NATIVE, a, l, false))
       return fail(idx, DDG_EINVAL, "build beam L too large for LDS");
     a.base = idx->rows;
     a.graph = idx->graph;
@@ -1134,6 +1158,82 @@ ddg_status ddg_search(ddg_index* idx, const void* queries, uint64_t nq, const ui
   idx->prof[8] = tv;
   if (dist_cmps) std::memcpy(dist_cmps, cm.data(), nq * 8);
   if (visited) std::memcpy(visited, vs.data(), nq * 4);
+  return DDG_OK;
+}
+
+ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq, const ddg_query_params* qp,
+                               uint32_t* out4, double* ms2) {
+  if (!idx || !qp || (nq && !queries)) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nq == 0 || qp->beam == 0 || qp->k > qp->beam) return fail(idx, DDG_EINVAL, "bad query set or beam");
+  ddg_status st = prepare_trav(idx, (int)qp->trav, nullptr);
+  if (st != DDG_OK) return st;
+  const size_t row_bytes = (size_t)idx->ad * idx->es;
+  ENSURE(B_QUERIES, nq * row_bytes);
+  ENSURE(B_STARTS, 256);
+  ENSURE(B_OUT_IDS, nq * qp->k * 4 + 4);
+  ENSURE(B_VC, nq * qp->k * 4 + 4);
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_QUERIES), queries, nq * row_bytes, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(launch_pad_rows(buf<void>(idx, B_QUERIES), nq, idx->d * idx->es, (uint32_t)row_bytes, idx->stream));
+  HIPCHK(launch_fill_u32(buf<uint32_t>(idx, B_STARTS), 0u, 1, idx->stream));
+  const uint32_t bits = filter_bits(qp->beam);
+  const uint32_t key = tune_key((int)qp->trav, bits);
+  // drop an earlier choice so the default shape is the reference point
+  for (size_t i = 0; i < idx->tuned.size(); i++)
+    if (idx->tuned[i].key == key) { idx->tuned.erase(idx->tuned.begin() + (long)i); break; }
+  auto time_shape = [&](const ddg_index::Tuned* cfg, uint32_t* out_ids, double* ms) -> ddg_status {
+    idx->cfg_override = cfg;
+    double t = 1e30;
+    for (int rep = 0; rep < 2; rep++) {  // a warm-up launch, then the timed one
+      const double before = idx->prof[0];
+      ddg_status s2 = ddg_search_device(idx, buf<void>(idx, B_QUERIES), nq, nullptr, buf<uint32_t>(idx, B_STARTS), 1,
+                                        qp, nullptr, qp->beam, out_ids, nullptr, nullptr, nullptr);
+      if (s2 != DDG_OK) { idx->cfg_override = nullptr; return s2; }
+      t = idx->prof[0] - before;
+    }
+    idx->cfg_override = nullptr;
+    *ms = t;
+    return DDG_OK;
+  };
+  double base_ms = 0.0;
+  st = time_shape(nullptr, buf<uint32_t>(idx, B_OUT_IDS), &base_ms);
+  if (st != DDG_OK) return st;
+  std::vector<uint32_t> ref(nq * qp->k), got(nq * qp->k);
+  HIPCHK(hipMemcpy(ref.data(), buf<void>(idx, B_OUT_IDS), ref.size() * 4, hipMemcpyDeviceToHost));
+  ddg_index::Tuned best{key, global_table(bits) ? 1 : 0, 0, 0};
+  double best_ms = base_ms;
+  uint32_t tried = 0;
+  for (int gtab = 0; gtab <= 1; gtab++) {
+    if (!gtab && (4u << bits) > 64u * 1024u) continue;  // LDS filters above 64 KB leave too few waves
+    for (uint32_t wpb : {1u, 2u, 4u})
+      for (uint32_t bpc : {1u, 2u, 3u, 4u, 6u, 8u}) {
+        if (wpb * bpc > 32) continue;
+        ddg_index::Tuned c{key, gtab, wpb, bpc};
+        SearchArgs a{};
+        Launch l{};
+        idx->cfg_override = &c;
+        const bool fits = search_layout(idx, qp->beam, 1, (int)qp->trav, a, l);
+        idx->cfg_override = nullptr;
+        if (!fits) continue;
+        double ms = 0.0;
+        st = time_shape(&c, buf<uint32_t>(idx, B_VC), &ms);
+        if (st != DDG_OK) return st;
+        tried++;
+        HIPCHK(hipMemcpy(got.data(), buf<void>(idx, B_VC), got.size() * 4, hipMemcpyDeviceToHost));
+        if (got != ref) return fail(idx, DDG_EDEVICE, "autotuner: a launch shape changed the search result");
+        if (ms < best_ms) { best_ms = ms; best = c; }
+      }
+  }
+  if (best.wpb) idx->tuned.push_back(best);
+  if (out4) {
+    SearchArgs a{};
+    Launch l{};
+    search_layout(idx, qp->beam, 1, (int)qp->trav, a, l);
+    out4[0] = a.gtable_words ? 1u : 0u;
+    out4[1] = l.waves_per_block;
+    out4[2] = l.blocks / idx->num_cu;
+    out4[3] = tried;
+  }
+  if (ms2) { ms2[0] = base_ms; ms2[1] = best_ms; }
   return DDG_OK;
 }
 

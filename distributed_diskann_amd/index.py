@@ -137,6 +137,19 @@ class Index:
                                           p(per_query_beam), beam_max, p(out_ids), p(out_dists), p(cmps),
                                           p(visited)))
 
+    def autotune_search(self, Q: np.ndarray, k=10, beam=50, cut=1.35, trav="native", rerank=0):
+        """Times the result-preserving search launch shapes on Q and keeps the
+        fastest for later searches with this traversal mode and beam class."""
+        Q = np.ascontiguousarray(Q)
+        if Q.dtype != self.rows.dtype or Q.shape[1] != self.ad:
+            raise ValueError("queries must match the base dtype and aligned dimension")
+        qp = QueryParamsC(k, beam, cut, 0, 0, TRAV[trav], rerank)
+        out = np.zeros(4, _U32)
+        ms = np.zeros(2, np.float64)
+        self._chk(lib().ddg_autotune_search(self.h, _c(Q), Q.shape[0], C.byref(qp), _c(out), _c(ms)))
+        return {"global_filter": bool(out[0]), "waves_per_block": int(out[1]), "blocks_per_cu": int(out[2]),
+                "tried": int(out[3]), "default_ms": float(ms[0]), "tuned_ms": float(ms[1])}
+
     def prepare_traversal(self, trav):
         """Builds the BF16 / FP8 traversal copy; returns its device seconds."""
         sec = C.c_double(0.0)

@@ -158,6 +158,18 @@ ddg_status ddg_sort_rows(ddg_index* idx);
  * counterpart: the reference traverses its float32 rows only. */
 ddg_status ddg_prepare_traversal(ddg_index* idx, ddg_trav trav, double* seconds);
 
+/* Autotuner (no reference counterpart): times the search launch shapes that
+ * give the same results -- visited filter in LDS or in global memory, 1/2/4
+ * waves per block, 1..8 resident blocks per CU -- on these queries
+ * (nq*aligned_d, host memory) with qp, checks that every shape returns the
+ * same ids, and keeps the fastest for this handle's later searches with the
+ * same traversal mode and filter size.  The filter size and frontier width
+ * are fixed by beamSearch.h:64-73 and the beam, so they are not tuned.
+ * out4 (nullable): [filter in global memory, waves per block, blocks per CU,
+ * shapes tried]; ms2 (nullable): [default shape ms, chosen shape ms]. */
+ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq, const ddg_query_params* qp,
+                               uint32_t* out4, double* ms2);
+
 /* Point::distance (euclidian_point.h:44-65, mips_point.h:44-66) batched, in
  * the arithmetic of traversal mode trav: out[i*m + j] = distance of query i
  * (nq*aligned_d rows) to point ids[i*m + j].  NATIVE gives the reference's

@@ -55,6 +55,24 @@ def test_search_large_beams(oracle, xs, Ls):
     assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
 
 
+@pytest.mark.parametrize("Ls", [50, 120])
+def test_autotune_search(oracle, xs, Ls):
+    """The autotuner only picks among result-preserving launch shapes: after
+    tuning, searches still equal the oracle bit for bit."""
+    print(f"[gpu] autotune Ls={Ls}", flush=True)
+    idx = _index(xs["X"], xs["d"], xs["G"])
+    info = idx.autotune_search(xs["Q"], 10, Ls)
+    print(f"[gpu] autotune Ls={Ls}: {info}", flush=True)
+    assert info["tried"] >= 4
+    assert info["tuned_ms"] <= info["default_ms"]
+    ids, dists, cmps, vis = idx.search(xs["Q"], 10, Ls, 1.35)
+    P = oracle.Points(xs["X"], xs["d"])
+    oi, od, oc, ov = oracle.search_all(P, oracle.Graph(xs["G"]), xs["Q"], 10, Ls, cut=1.35)
+    assert np.array_equal(ids, oi)
+    assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+    assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+
+
 def test_search_per_query_beam(oracle, xs):
     print("[gpu] per-query beam", flush=True)
     rng = np.random.default_rng(7)
