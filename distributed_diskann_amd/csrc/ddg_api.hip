@@ -63,7 +63,7 @@ struct ddg_index {
   struct Tuned {
     uint32_t key;
     int gtab;
-    uint32_t wpb, bpc;
+    uint32_t wpb, bpc, stage;  // stage: rows gathered into LDS per batch (0 = one lane per row)
   };
   std::vector<Tuned> tuned;
   const Tuned* cfg_override = nullptr;  // set while the autotuner times a shape
@@ -208,6 +208,19 @@ static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nsta
   a.off_cand = off; off += rcap * 8;
   a.off_sorted = off; off += rcap * 8;
   a.off_bkt = off; off += 1024;
+  // rows of more than one load batch (8 chunks of 16 B) can be gathered into
+  // LDS, cfg->stage rows at a time (at most 512 chunks, one batch per lane;
+  // stride odd in chunks); the LDS this takes costs occupancy, so only a
+  // tuned shape turns it on
+  const uint32_t C = trav == DDG_TRAV_NATIVE ? (idx->ad * idx->es) / 16 : a.trav_chunks;
+  a.stage_rows = 0;
+  if (cfg && cfg->stage && C > 8) {
+    a.stage_chunks = C;
+    a.stage_stride = C % 2 ? C : C + 1;
+    a.stage_rows = std::max<uint32_t>(1, std::min<uint32_t>(std::min<uint32_t>(64, 512 / C), cfg->stage));
+    a.off_stage = round_up(off, 16);
+    off = a.off_stage + a.stage_rows * a.stage_stride * 16;
+  }
   a.lds_per_wave = round_up(off, 16);
   a.fcap = fcap;
   a.rcap = rcap;
@@ -1199,7 +1212,7 @@ ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq,
   if (st != DDG_OK) return st;
   std::vector<uint32_t> ref(nq * qp->k), got(nq * qp->k);
   HIPCHK(hipMemcpy(ref.data(), buf<void>(idx, B_OUT_IDS), ref.size() * 4, hipMemcpyDeviceToHost));
-  ddg_index::Tuned best{key, global_table(bits) ? 1 : 0, 0, 0};
+  ddg_index::Tuned best{key, global_table(bits) ? 1 : 0, 0, 0, 0};
   double best_ms = base_ms;
   uint32_t tried = 0;
   for (int gtab = 0; gtab <= 1; gtab++) {
@@ -1207,7 +1220,7 @@ ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq,
     for (uint32_t wpb : {1u, 2u, 4u})
       for (uint32_t bpc : {1u, 2u, 3u, 4u, 6u, 8u}) {
         if (wpb * bpc > 32) continue;
-        ddg_index::Tuned c{key, gtab, wpb, bpc};
+        ddg_index::Tuned c{key, gtab, wpb, bpc, 0};
         SearchArgs a{};
         Launch l{};
         idx->cfg_override = &c;
@@ -1223,12 +1236,38 @@ ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq,
         if (ms < best_ms) { best_ms = ms; best = c; }
       }
   }
+  // long rows: with the best shape, try gathering 2 / 4 / 8 / as many rows as
+  // one load batch holds into LDS per round trip
+  const uint32_t C = qp->trav == DDG_TRAV_NATIVE ? (idx->ad * idx->es) / 16 : (idx->d * trav_esize(idx, (int)qp->trav) + 15) / 16;
+  if (C > 8 && best.wpb) {
+    const uint32_t smax = std::max<uint32_t>(1, std::min<uint32_t>(64, 512 / C));
+    std::vector<uint32_t> stages;
+    for (uint32_t sr : {2u, 4u, 8u, smax})
+      if (sr <= smax && std::find(stages.begin(), stages.end(), sr) == stages.end()) stages.push_back(sr);
+    for (uint32_t sr : stages) {
+      ddg_index::Tuned c = best;
+      c.stage = sr;
+      SearchArgs a{};
+      Launch l{};
+      idx->cfg_override = &c;
+      const bool fits = search_layout(idx, qp->beam, 1, (int)qp->trav, a, l);
+      idx->cfg_override = nullptr;
+      if (!fits) continue;
+      double ms = 0.0;
+      st = time_shape(&c, buf<uint32_t>(idx, B_VC), &ms);
+      if (st != DDG_OK) return st;
+      tried++;
+      HIPCHK(hipMemcpy(got.data(), buf<void>(idx, B_VC), got.size() * 4, hipMemcpyDeviceToHost));
+      if (got != ref) return fail(idx, DDG_EDEVICE, "autotuner: a launch shape changed the search result");
+      if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+  }
   if (best.wpb) idx->tuned.push_back(best);
   if (out4) {
     SearchArgs a{};
     Launch l{};
     search_layout(idx, qp->beam, 1, (int)qp->trav, a, l);
-    out4[0] = a.gtable_words ? 1u : 0u;
+    out4[0] = (a.gtable_words ? 1u : 0u) | (a.stage_rows << 8);
     out4[1] = l.waves_per_block;
     out4[2] = l.blocks / idx->num_cu;
     out4[3] = tried;
@@ -1306,6 +1345,18 @@ ddg_status ddg_insert_search(ddg_index* idx, const uint32_t* ids, uint64_t m, ui
     std::memcpy(vis_ids + i * vcap, vi.data() + i * cap, vis_len[i] * 4);
     std::memcpy(vis_dists + i * vcap, vd.data() + i * cap, vis_len[i] * 4);
   }
+  return DDG_OK;
+}
+
+// Test hook (not part of include/ddg.h): records a search launch shape for
+// (trav, beam) as the autotuner would, so tests can run every shape.
+ddg_status ddg_internal_set_shape(ddg_index* idx, int trav, uint32_t beam, int gtab, uint32_t wpb, uint32_t bpc,
+                                  uint32_t stage) {
+  if (!idx || wpb == 0 || bpc == 0 || wpb > 4 || wpb * bpc > 32) return fail(idx, DDG_EINVAL, "bad shape");
+  const uint32_t key = tune_key(trav, filter_bits(beam));
+  for (size_t i = 0; i < idx->tuned.size(); i++)
+    if (idx->tuned[i].key == key) { idx->tuned.erase(idx->tuned.begin() + (long)i); break; }
+  idx->tuned.push_back(ddg_index::Tuned{key, gtab, wpb, bpc, stage});
   return DDG_OK;
 }
 

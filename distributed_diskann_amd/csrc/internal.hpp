@@ -39,6 +39,10 @@ struct SearchArgs {
   // resident wave, indexed by block * waves_per_block + wave
   uint32_t* gtable;
   uint32_t gtable_words;
+  // rows longer than one load batch are gathered into LDS cooperatively:
+  // stage_rows rows of stage_chunks 16-byte chunks, stage_stride apart
+  // (chunks), at off_stage; stage_rows = 0 keeps one lane per row
+  uint32_t stage_rows, stage_chunks, stage_stride, off_stage;
   // per-wave LDS layout (bytes); the staged query takes qrow_bytes at offset
   // 0, its FP8 factors another qrow_bytes at off_qaux
   uint32_t lds_per_wave, qrow_bytes, off_qaux, off_table, off_f0, off_f1, off_keep, off_cand, off_sorted, off_bkt;

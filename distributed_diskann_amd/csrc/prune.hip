@@ -164,7 +164,7 @@ template <typename T, int METRIC, bool BIG>
 __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t wave = bcast0(threadIdx.x >> 6);  // scalar: per-wave pointers stay in SGPRs
   unsigned char* my = smem + wave * a.lds_per_wave;
   T* pv = (T*)my;                                   // p's row
   T* sv = (T*)(my + ((a.ad * sizeof(T) + 15u) & ~15u));  // p*'s row

@@ -41,9 +41,12 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using P = TravDist<T, METRIC, TRAV>;
   const uint32_t lane = lane_id();
-  unsigned char* my = smem + (threadIdx.x >> 6) * a.lds_per_wave;
+  // the wave's index in its block, as a scalar: every per-wave pointer below
+  // is then wave-uniform (SGPRs), not per-lane
+  const uint32_t wv = bcast0(threadIdx.x >> 6);
+  unsigned char* my = smem + wv * a.lds_per_wave;
   unsigned char* ql = my;  // staged query
-  uint32_t* table = GTAB ? a.gtable + ((uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * a.gtable_words
+  uint32_t* table = GTAB ? a.gtable + ((uint64_t)blockIdx.x * (blockDim.x >> 6) + wv) * a.gtable_words
                         : (uint32_t*)(my + a.off_table);
   float* Fd0 = (float*)(my + a.off_f0);
   uint32_t* Fi0 = (uint32_t*)(Fd0 + a.fcap);
@@ -224,24 +227,73 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       // ---- distances and cutoff (beamSearch.h:135-145)
       const float cutoff = (fsz < beam) ? 2147483648.0f : FD[fsz - 1];
       uint32_t nc = 0;
-      for (uint32_t b0 = 0; b0 < nk; b0 += kWave) {
-        const uint32_t j = b0 + lane;
-        const bool v = j < nk;
-        uint32_t id = 0;
-        float dd = 0.f;
-        if (v) {
-          id = Kid[j];
-          dd = P::dist(a, id, ql, qs);
+      if (a.stage_rows) {
+        // Long rows: the wave gathers stage_rows candidate rows into LDS
+        // together (all lanes' loads in flight at once), then lane j computes
+        // row j's distance from LDS with the same arithmetic.
+        const uint32_t C = a.stage_chunks, CS = a.stage_stride;
+        uint4* stg = (uint4*)(my + a.off_stage);
+        for (uint32_t r0 = 0; r0 < nk; r0 += a.stage_rows) {
+          const uint32_t nr = nk - r0 < a.stage_rows ? nk - r0 : a.stage_rows;
+          const uint32_t total = nr * C;
+          for (uint32_t e0 = 0; e0 < total; e0 += kChunkBatch * kWave) {
+            uint4 xs[kChunkBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunkBatch; u++) {
+              const uint32_t e = e0 + u * kWave + lane;
+              if (e < total) {
+                const uint32_t r = e / C, c = e - r * C;
+                xs[u] = P::row_ptr(a, Kid[r0 + r])[c];
+              }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kChunkBatch; u++) {
+              const uint32_t e = e0 + u * kWave + lane;
+              if (e < total) {
+                const uint32_t r = e / C, c = e - r * C;
+                stg[r * CS + c] = xs[u];
+              }
+            }
+          }
+          wave_sync();
+          const bool v = lane < nr;
+          uint32_t id = 0;
+          float dd = 0.f;
+          if (v) {
+            id = Kid[r0 + lane];
+            dd = P::dist_row(a, stg + lane * CS, ql, qs);
+          }
+          const bool pass = v && !(dd >= cutoff);
+          const uint64_t pm = ballot(pass);
+          if (pass) {
+            const uint32_t o = nc + mbcnt(pm);
+            Cd[o] = dd;
+            Ci[o] = id;
+            if (less_di(dd, id, u_d, u_id)) { u_d = dd; u_id = id; }
+          }
+          nc += popc(pm);
+          wave_sync();  // the next batch overwrites the staged rows
         }
-        const bool pass = v && !(dd >= cutoff);
-        const uint64_t pm = ballot(pass);
-        if (pass) {
-          const uint32_t o = nc + mbcnt(pm);
-          Cd[o] = dd;
-          Ci[o] = id;
-          if (less_di(dd, id, u_d, u_id)) { u_d = dd; u_id = id; }
+      } else {
+        for (uint32_t b0 = 0; b0 < nk; b0 += kWave) {
+          const uint32_t j = b0 + lane;
+          const bool v = j < nk;
+          uint32_t id = 0;
+          float dd = 0.f;
+          if (v) {
+            id = Kid[j];
+            dd = P::dist(a, id, ql, qs);
+          }
+          const bool pass = v && !(dd >= cutoff);
+          const uint64_t pm = ballot(pass);
+          if (pass) {
+            const uint32_t o = nc + mbcnt(pm);
+            Cd[o] = dd;
+            Ci[o] = id;
+            if (less_di(dd, id, u_d, u_id)) { u_d = dd; u_id = id; }
+          }
+          nc += popc(pm);
         }
-        nc += popc(pm);
       }
       wave_sync();
       // predicted next vertex: the smallest of u and the new candidates
@@ -413,8 +465,9 @@ template <typename T, int METRIC, int TRAV>
 __global__ __launch_bounds__(256) void k_trav_dist(SearchArgs a, const uint32_t* ids, uint32_t m, float* out) {
   extern __shared__ __align__(16) unsigned char smem[];
   using P = TravDist<T, METRIC, TRAV>;
-  unsigned char* ql = smem + (threadIdx.x >> 6) * a.lds_per_wave;
-  const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const uint32_t wv = bcast0(threadIdx.x >> 6);
+  unsigned char* ql = smem + wv * a.lds_per_wave;
+  const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wv;
   if (q >= a.nq) return;  // whole waves leave together
   stage_traversal_query<T, METRIC, TRAV>(a, ql, (const T*)a.queries + q * a.ad);
   wave_sync();

@@ -28,6 +28,9 @@ enum TravMode : int { TRAV_NATIVE = 0, TRAV_BF16 = 1, TRAV_FP8 = 2 };
 template <typename T, int METRIC, int TRAV>
 struct TravDist;
 
+// Every policy offers: the row of point `id` as 16-byte chunks (row_ptr, for
+// a cooperative gather into LDS), and the distance to a row given by pointer
+// (in HBM or in LDS) or by id.
 template <typename T, int METRIC>
 struct TravDist<T, METRIC, TRAV_NATIVE> {
   using D = Dist<T, METRIC>;
@@ -35,8 +38,14 @@ struct TravDist<T, METRIC, TRAV_NATIVE> {
   __device__ static Q prep(const SearchArgs& a, const unsigned char* ql) {
     return D::prep((const T*)ql, (a.ad * (uint32_t)sizeof(T)) >> 4);
   }
+  __device__ static const uint4* row_ptr(const SearchArgs& a, uint32_t id) {
+    return (const uint4*)((const T*)a.base + (uint64_t)id * a.ad);
+  }
+  __device__ static float dist_row(const SearchArgs& a, const uint4* row, const unsigned char* ql, const Q& qs) {
+    return D::dist((const T*)row, (const T*)ql, qs, a.d, (a.ad * (uint32_t)sizeof(T)) >> 4);
+  }
   __device__ static float dist(const SearchArgs& a, uint32_t id, const unsigned char* ql, const Q& qs) {
-    return D::dist((const T*)a.base + (uint64_t)id * a.ad, (const T*)ql, qs, a.d, (a.ad * (uint32_t)sizeof(T)) >> 4);
+    return dist_row(a, row_ptr(a, id), ql, qs);
   }
 };
 
@@ -57,8 +66,13 @@ template <int METRIC>
 struct TravDist<float, METRIC, TRAV_BF16> {
   struct Q { int unused; };
   __device__ static Q prep(const SearchArgs&, const unsigned char*) { return Q{0}; }
-  __device__ static float dist(const SearchArgs& a, uint32_t id, const unsigned char* ql, const Q&) {
-    const uint4* r = (const uint4*)((const unsigned char*)a.trav_rows + (uint64_t)id * a.trav_stride);
+  __device__ static const uint4* row_ptr(const SearchArgs& a, uint32_t id) {
+    return (const uint4*)((const unsigned char*)a.trav_rows + (uint64_t)id * a.trav_stride);
+  }
+  __device__ static float dist(const SearchArgs& a, uint32_t id, const unsigned char* ql, const Q& qs) {
+    return dist_row(a, row_ptr(a, id), ql, qs);
+  }
+  __device__ static float dist_row(const SearchArgs& a, const uint4* r, const unsigned char* ql, const Q&) {
     const float4* q = (const float4*)ql;
     float s0 = 0.f, s1 = 0.f;
     for (uint32_t c0 = 0; c0 < a.trav_chunks; c0 += kChunkBatch) {
@@ -112,8 +126,13 @@ template <int METRIC>
 struct TravDist<float, METRIC, TRAV_FP8> {
   struct Q { int unused; };
   __device__ static Q prep(const SearchArgs&, const unsigned char*) { return Q{0}; }
-  __device__ static float dist(const SearchArgs& a, uint32_t id, const unsigned char* ql, const Q&) {
-    const uint4* r = (const uint4*)((const unsigned char*)a.trav_rows + (uint64_t)id * a.trav_stride);
+  __device__ static const uint4* row_ptr(const SearchArgs& a, uint32_t id) {
+    return (const uint4*)((const unsigned char*)a.trav_rows + (uint64_t)id * a.trav_stride);
+  }
+  __device__ static float dist(const SearchArgs& a, uint32_t id, const unsigned char* ql, const Q& qs) {
+    return dist_row(a, row_ptr(a, id), ql, qs);
+  }
+  __device__ static float dist_row(const SearchArgs& a, const uint4* r, const unsigned char* ql, const Q&) {
     const float4* q = (const float4*)ql;
     const float4* sc = (const float4*)(ql + a.off_qaux);
     float s0 = 0.f, s1 = 0.f;

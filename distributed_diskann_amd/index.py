@@ -147,8 +147,9 @@ class Index:
         out = np.zeros(4, _U32)
         ms = np.zeros(2, np.float64)
         self._chk(lib().ddg_autotune_search(self.h, _c(Q), Q.shape[0], C.byref(qp), _c(out), _c(ms)))
-        return {"global_filter": bool(out[0]), "waves_per_block": int(out[1]), "blocks_per_cu": int(out[2]),
-                "tried": int(out[3]), "default_ms": float(ms[0]), "tuned_ms": float(ms[1])}
+        return {"global_filter": bool(out[0] & 1), "staged_rows": int(out[0] >> 8), "waves_per_block": int(out[1]),
+                "blocks_per_cu": int(out[2]), "tried": int(out[3]), "default_ms": float(ms[0]),
+                "tuned_ms": float(ms[1])}
 
     def prepare_traversal(self, trav):
         """Builds the BF16 / FP8 traversal copy; returns its device seconds."""

@@ -165,8 +165,11 @@ ddg_status ddg_prepare_traversal(ddg_index* idx, ddg_trav trav, double* seconds)
  * same ids, and keeps the fastest for this handle's later searches with the
  * same traversal mode and filter size.  The filter size and frontier width
  * are fixed by beamSearch.h:64-73 and the beam, so they are not tuned.
- * out4 (nullable): [filter in global memory, waves per block, blocks per CU,
- * shapes tried]; ms2 (nullable): [default shape ms, chosen shape ms]. */
+ * For rows longer than 128 bytes it also tries gathering 2, 4, 8 or as
+ * many candidate rows as one load batch holds into LDS per round trip.
+ * out4 (nullable): [filter in global memory (bit 0) | staged rows << 8,
+ * waves per block, blocks per CU, shapes tried]; ms2 (nullable): [default
+ * shape ms, chosen shape ms]. */
 ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq, const ddg_query_params* qp,
                                uint32_t* out4, double* ms2);
 

@@ -73,6 +73,30 @@ def test_autotune_search(oracle, xs, Ls):
     assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
 
 
+@pytest.mark.parametrize("stage", [2, 9])
+@pytest.mark.parametrize("dtype,metric,d", [("float32", "l2", 200), ("float32", "mips", 150)])
+def test_search_staged_rows(oracle, dtype, metric, d, stage):
+    """Long rows gathered into LDS (stage rows per round trip): same bits as
+    one lane per row, for the native and the reduced traversals."""
+    print(f"[gpu] staged rows {metric} d={d} stage={stage}", flush=True)
+    import ctypes as C
+    from distributed_diskann_amd._lib import lib
+    Xa, Qa, P, G = _small_case(oracle, dtype, metric, 3000, d, seed=d + stage)
+    idx = _index(Xa, d, G.rows, metric)
+    base = {m: idx.search(Qa, 10, 60, 1.35, trav=m) for m in ("native", "bf16", "fp8")}
+    f = lib().ddg_internal_set_shape
+    f.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+    f.restype = C.c_int
+    for t, m in enumerate(("native", "bf16", "fp8")):
+        assert f(idx.h, t, 60, 1, 4, 2, stage) == 0
+        ids, dists, cmps, vis = idx.search(Qa, 10, 60, 1.35, trav=m)
+        b = base[m]
+        assert np.array_equal(ids, b[0]) and np.array_equal(dists.view(np.uint32), b[1].view(np.uint32))
+        assert np.array_equal(cmps, b[2]) and np.array_equal(vis, b[3])
+    oi, od, oc, ov = oracle.search_all(P, G, Qa, 10, 60, cut=1.35)
+    assert np.array_equal(base["native"][0], oi) and np.array_equal(base["native"][2], oc)
+
+
 def test_search_per_query_beam(oracle, xs):
     print("[gpu] per-query beam", flush=True)
     rng = np.random.default_rng(7)
