@@ -206,7 +206,7 @@ static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nsta
   a.off_f1 = off; off += fcap * 8;
   a.off_keep = off; off += rcap * 4;
   a.off_cand = off; off += rcap * 8;
-  a.off_sorted = off; off += rcap * 8;
+  a.off_aux = off; off += rcap * 8;
   a.off_bkt = off; off += 1024;
   // rows of more than one load batch (8 chunks of 16 B) can be gathered into
   // LDS, cfg->stage rows at a time (at most 512 chunks, one batch per lane;

@@ -45,7 +45,7 @@ struct SearchArgs {
   uint32_t stage_rows, stage_chunks, stage_stride, off_stage;
   // per-wave LDS layout (bytes); the staged query takes qrow_bytes at offset
   // 0, its FP8 factors another qrow_bytes at off_qaux
-  uint32_t lds_per_wave, qrow_bytes, off_qaux, off_table, off_f0, off_f1, off_keep, off_cand, off_sorted, off_bkt;
+  uint32_t lds_per_wave, qrow_bytes, off_qaux, off_table, off_f0, off_f1, off_keep, off_cand, off_aux, off_bkt;
   uint32_t fcap, rcap;
 };
 

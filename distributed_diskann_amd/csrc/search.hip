@@ -3,7 +3,7 @@
 // by a device work counter.
 //
 // Per-wave LDS:  [query row][visited filter 2^bits u32][frontier A][frontier B]
-//                [kept ids][candidates][sorted candidates][slot buckets, 1 KB]
+//                [kept ids][candidates][candidate ranks][slot buckets, 1 KB]
 // With GTAB the visited filter lives in global memory instead, one slice of
 // 2^bits words per resident wave (large beams: the LDS then holds only a few
 // KB per wave, and many more waves fit on a CU).
@@ -36,6 +36,40 @@ namespace ddg {
 
 constexpr uint32_t kBuckets = 1024;  // slot buckets for the lane-conflict check
 
+// Frontier and candidate entries are (distance bits, id) pairs; bit 31 of a
+// frontier id marks a visited entry.  Order: (distance, id) of
+// beamSearch.h:59-61, ignoring the flag.
+__device__ __forceinline__ bool less_e(uint2 a, uint2 b) {
+  return less_di(__uint_as_float(a.x), a.y & kIdMask, __uint_as_float(b.x), b.y & kIdMask);
+}
+__device__ __forceinline__ bool same_e(uint2 a, uint2 b) {
+  return __uint_as_float(a.x) == __uint_as_float(b.x) && (a.y & kIdMask) == (b.y & kIdMask);
+}
+
+// Number of entries of the sorted list F[0, n) strictly below c: binary
+// lifting over powers of two, the same number of steps on every lane (n is
+// wave-uniform).
+__device__ __forceinline__ uint32_t count_below(const uint2* F, uint32_t n, uint2 c) {
+  uint32_t lo = 0;
+  for (uint32_t step = n ? 1u << (31 - __builtin_clz(n)) : 0u; step; step >>= 1) {
+    const uint32_t probe = lo + step;
+    const uint2 f = F[(probe <= n ? probe : n) - 1u];
+    if (probe <= n && less_e(f, c)) lo = probe;
+  }
+  return lo;
+}
+
+// Number of entries of the ascending list U[0, n) not above x.
+__device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t n, uint32_t x) {
+  uint32_t lo = 0;
+  for (uint32_t step = n ? 1u << (31 - __builtin_clz(n)) : 0u; step; step >>= 1) {
+    const uint32_t probe = lo + step;
+    const uint32_t u = U[(probe <= n ? probe : n) - 1u];
+    if (probe <= n && u <= x) lo = probe;
+  }
+  return lo;
+}
+
 template <typename T, int METRIC, int TRAV, bool GTAB>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -48,16 +82,13 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
   unsigned char* ql = my;  // staged query
   uint32_t* table = GTAB ? a.gtable + ((uint64_t)blockIdx.x * (blockDim.x >> 6) + wv) * a.gtable_words
                         : (uint32_t*)(my + a.off_table);
-  float* Fd0 = (float*)(my + a.off_f0);
-  uint32_t* Fi0 = (uint32_t*)(Fd0 + a.fcap);
-  float* Fd1 = (float*)(my + a.off_f1);
-  uint32_t* Fi1 = (uint32_t*)(Fd1 + a.fcap);
-  uint32_t* Kid = (uint32_t*)(my + a.off_keep);
+  uint2* F0 = (uint2*)(my + a.off_f0);
+  uint2* F1 = (uint2*)(my + a.off_f1);
+  uint32_t* Kid = (uint32_t*)(my + a.off_keep);  // kept neighbor ids; later the candidates' ub list
   uint8_t* bkt = (uint8_t*)(my + a.off_bkt);
-  float* Cd = (float*)(my + a.off_cand);
-  uint32_t* Ci = (uint32_t*)(Cd + a.rcap);
-  float* Sd = (float*)(my + a.off_sorted);
-  uint32_t* Si = (uint32_t*)(Sd + a.rcap);
+  uint2* Cn = (uint2*)(my + a.off_cand);   // candidates of one expansion, in neighbor order
+  uint2* Ax = (uint2*)(my + a.off_aux);    // per candidate: (rank, entries of F not above it)
+  uint32_t* Ub = Kid;
 
   const uint64_t R1 = (uint64_t)a.R + 1;
   const T* base = (const T*)a.base;
@@ -91,24 +122,20 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     // starting frontier, sorted by (distance, id)
     const uint32_t ns = a.nstarts;
     {
-      float sd = 0.f;
-      uint32_t sid = 0;
+      uint2 se = make_uint2(0u, 0u);
       if (lane < ns) {
-        sid = a.starts[lane];
-        sd = P::dist(a, sid, ql, qs);
-        Cd[lane] = sd;
-        Ci[lane] = sid;
+        const uint32_t sid = a.starts[lane];
+        se = make_uint2(__float_as_uint(P::dist(a, sid, ql, qs)), sid);
+        Cn[lane] = se;
       }
       wave_sync();
       if (lane < ns) {
         uint32_t rank = 0;
         for (uint32_t t = 0; t < ns; t++) {
-          float dt = Cd[t];
-          uint32_t it = Ci[t];
-          rank += (less_di(dt, it, sd, sid) || (dt == sd && it == sid && t < lane)) ? 1u : 0u;
+          const uint2 st = Cn[t];
+          rank += (less_e(st, se) || (same_e(st, se) && t < lane)) ? 1u : 0u;
         }
-        Fd0[rank] = sd;
-        Fi0[rank] = sid;
+        F0[rank] = se;
       }
       wave_sync();
     }
@@ -117,6 +144,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     uint64_t cmps = ns;
     uint64_t nvis = 0;
     int pos = fsz > 0 ? 0 : -1;
+    bool trimmed = false;  // the frontier has been through one union + trim
     // Adjacency row of the vertex predicted to be expanded next, loaded while
     // the current expansion is merged (degree and the first 64 neighbors).
     // The prediction (the smallest unvisited entry of frontier and new
@@ -124,14 +152,14 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     // continues.  With a global filter the table words of those neighbors are
     // loaded ahead too: no table store happens between that load and the
     // next expansion's filter, so they are the words the filter would read.
-    uint32_t pf_id = 0xFFFFFFFFu, pf_deg = 0, pf_nb = 0, pf_prev = 0xFFFFFFFFu;
+    uint32_t pf_id = 0xFFFFFFFFu, pf_deg = 0, pf_nb = 0, pf_prev = 0xFFFFFFFFu, pf_slot = 0;
     bool pf_table = false;
 
     while (pos >= 0 && nvis < a.limit) {
-      float* FD = cur ? Fd1 : Fd0;
-      uint32_t* FI = cur ? Fi1 : Fi0;
-      const float cdist = FD[pos];
-      const uint32_t cid = FI[pos] & kIdMask;
+      uint2* FR = cur ? F1 : F0;
+      const uint2 ce = FR[pos];
+      const float cdist = __uint_as_float(ce.x);
+      const uint32_t cid = ce.y & kIdMask;
       const uint32_t* row = a.graph + (uint64_t)cid * R1;
       uint32_t deg, nb0;
       const bool use_pf_table = GTAB && pf_table && pf_id == cid;
@@ -144,7 +172,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       }
       wave_sync();
       if (lane == 0) {
-        FI[pos] = cid | kFlag;
+        ((uint32_t*)FR)[2 * pos + 1] = cid | kFlag;
         if (a.build_mode) {
           if (nvis < a.vcap) {
             a.vis_ids[(uint64_t)q * a.vcap + nvis] = cid;
@@ -156,22 +184,19 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       }
       nvis++;
 
-      // the first unvisited frontier entry after cur: the next vertex to
-      // expand unless a new candidate beats it
-      wave_sync();
-      float u_d = __builtin_inff();
-      uint32_t u_id = 0xFFFFFFFFu;
-      for (uint32_t b0 = 0; b0 < fsz; b0 += kWave) {
+      // the first unvisited frontier entry after pos (every entry before pos
+      // is visited): the next vertex to expand unless a new candidate beats it
+      int ui = -1;
+      for (uint32_t b0 = (uint32_t)pos + 1u; b0 < fsz; b0 += kWave) {
         const uint32_t i = b0 + lane;
-        const bool un = i < fsz && !(FI[i] & kFlag);
+        const bool un = i < fsz && !(FR[i].y & kFlag);
         const uint64_t um = ballot(un);
         if (um) {
-          const uint32_t ui = b0 + (uint32_t)first_lane(um);
-          u_d = FD[ui];
-          u_id = FI[ui] & kIdMask;
+          ui = (int)(b0 + (uint32_t)first_lane(um));
           break;
         }
       }
+      const uint2 ue = ui >= 0 ? FR[ui] : make_uint2(0u, 0u);
 
       // ---- neighbor filter (beamSearch.h:125-131)
       const uint32_t m = deg < a.degree_limit ? deg : a.degree_limit;
@@ -181,10 +206,11 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         const bool valid = j < m;
         const uint32_t av = valid ? (b0 == 0 ? nb0 : row[1 + j]) : 0u;
         const bool act = valid && ((int64_t)av != self_id);
-        const uint32_t slot = (uint32_t)hash64_2((uint64_t)av) & mask;
+        const bool pre = b0 == 0 && use_pf_table;  // slot and table word loaded ahead
+        const uint32_t slot = pre ? pf_slot : (uint32_t)hash64_2((uint64_t)av) & mask;
         if constexpr (GTAB) wave_sync_global();  // earlier table stores of other lanes
         uint32_t prev = 0xFFFFFFFFu;
-        if (act) prev = (b0 == 0 && use_pf_table) ? pf_prev : table[slot];
+        if (act) prev = pre ? pf_prev : table[slot];
         bool has_next = false;
         // Lanes that share a slot share a bucket of `bkt`.  Every active lane
         // writes its lane id into its bucket; a lane that reads back another
@@ -225,7 +251,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       cmps += nk;
 
       // ---- distances and cutoff (beamSearch.h:135-145)
-      const float cutoff = (fsz < beam) ? 2147483648.0f : FD[fsz - 1];
+      const float cutoff = (fsz < beam) ? 2147483648.0f : __uint_as_float(FR[fsz - 1].x);
       uint32_t nc = 0;
       if (a.stage_rows) {
         // Long rows: the wave gathers stage_rows candidate rows into LDS
@@ -265,12 +291,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
           }
           const bool pass = v && !(dd >= cutoff);
           const uint64_t pm = ballot(pass);
-          if (pass) {
-            const uint32_t o = nc + mbcnt(pm);
-            Cd[o] = dd;
-            Ci[o] = id;
-            if (less_di(dd, id, u_d, u_id)) { u_d = dd; u_id = id; }
-          }
+          if (pass) Cn[nc + mbcnt(pm)] = make_uint2(__float_as_uint(dd), id);
           nc += popc(pm);
           wave_sync();  // the next batch overwrites the staged rows
         }
@@ -286,156 +307,181 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
           }
           const bool pass = v && !(dd >= cutoff);
           const uint64_t pm = ballot(pass);
-          if (pass) {
-            const uint32_t o = nc + mbcnt(pm);
-            Cd[o] = dd;
-            Ci[o] = id;
-            if (less_di(dd, id, u_d, u_id)) { u_d = dd; u_id = id; }
-          }
+          if (pass) Cn[nc + mbcnt(pm)] = make_uint2(__float_as_uint(dd), id);
           nc += popc(pm);
         }
       }
       wave_sync();
-      // predicted next vertex: the smallest of u and the new candidates
-      for (int o = 1; o < (int)kWave; o <<= 1) {
-        const float od = __shfl_xor(u_d, o);
-        const uint32_t oi = __shfl_xor(u_id, o);
-        if (less_di(od, oi, u_d, u_id)) { u_d = od; u_id = oi; }
+
+      // ---- order of the candidates (std::sort, beamSearch.h:148): the rank
+      // of candidate j counts the candidates below it plus its equal copies at
+      // lower indices; the smallest one (rank 0) is the prediction's rival
+      bool have_min = false;
+      uint2 cmin = make_uint2(0u, 0u);
+      for (uint32_t cb = 0; cb < nc; cb += kWave) {
+        const uint32_t j = cb + lane;
+        const bool v = j < nc;
+        uint2 cj = make_uint2(0u, 0u);
+        uint32_t lt = 0, eqb = 0;
+        if (v) {
+          cj = Cn[j];
+          for (uint32_t t = 0; t < nc; t++) {
+            const uint2 ct = Cn[t];
+            lt += less_e(ct, cj) ? 1u : 0u;
+            eqb += (t < j && same_e(ct, cj)) ? 1u : 0u;
+          }
+          Ax[j].x = lt + eqb;
+        }
+        const uint64_t zm = ballot(v && lt + eqb == 0u);
+        if (zm) {
+          const int l = first_lane(zm);
+          cmin = make_uint2(readlane_u32(cj.x, l), readlane_u32(cj.y, l));
+          have_min = true;
+        }
       }
-      pf_id = bcast0(u_id);
+      // predicted next vertex: the smaller of the first unvisited entry and
+      // the smallest candidate; its adjacency row is loaded during the merge
+      pf_id = (have_min && (ui < 0 || less_e(cmin, ue))) ? cmin.y : (ui >= 0 ? (ue.y & kIdMask) : 0xFFFFFFFFu);
       if (pf_id != 0xFFFFFFFFu) {
         const uint32_t* prow = a.graph + (uint64_t)pf_id * R1;
         pf_nb = lane < a.R ? prow[1 + lane] : 0u;
         pf_deg = prow[0];
       }
 
+      // ---- multiset union (beamSearch.h:151-154): a candidate equal to e
+      // frontier entries is emitted only for its copies beyond e.  For each
+      // candidate: ub = #frontier entries not above it.
+      uint32_t ne = 0;
+      for (uint32_t cb = 0; cb < nc; cb += kWave) {
+        const uint32_t j = cb + lane;
+        const bool v = j < nc;
+        bool emit = false;
+        if (v) {
+          const uint2 cj = Cn[j];
+          const uint32_t rank = Ax[j].x;
+          const uint32_t lb = count_below(FR, fsz, cj);
+          uint32_t eq = 0;
+          while (lb + eq < fsz && same_e(FR[lb + eq], cj)) eq++;
+          // equal copies at lower indices: rank - #candidates strictly below
+          uint32_t lt = 0;
+          if (eq > 0)
+            for (uint32_t t = 0; t < nc; t++) lt += less_e(Cn[t], cj) ? 1u : 0u;
+          emit = eq == 0 || rank - lt >= eq;
+          Ax[j] = make_uint2(emit ? rank : (rank | kFlag), lb + eq);
+        }
+        ne += popc(ballot(emit));
+      }
+      wave_sync();
+
       uint32_t nsz = fsz;
-      if (nc > 0) {
-        // ---- sort candidates by (distance, id) (beamSearch.h:148)
-        for (uint32_t j = lane; j < nc; j += kWave) {
-          const float dj = Cd[j];
-          const uint32_t ij = Ci[j];
-          uint32_t rank = 0;
-          for (uint32_t t = 0; t < nc; t++) {
-            const float dt = Cd[t];
-            const uint32_t itt = Ci[t];
-            rank += (less_di(dt, itt, dj, ij) || (dt == dj && itt == ij && t < j)) ? 1u : 0u;
-          }
-          Sd[rank] = dj;
-          Si[rank] = ij;
-        }
-        wave_sync();
-        // ---- multiset union: keep the copies beyond those already in F
-        uint32_t ne = 0;
-        for (uint32_t b0 = 0; b0 < nc; b0 += kWave) {
-          const uint32_t j = b0 + lane;
-          const bool v = j < nc;
-          bool emit = false;
-          float dj = 0.f;
-          uint32_t ij = 0;
-          if (v) {
-            dj = Sd[j];
-            ij = Si[j];
-            const uint32_t lb = lower_bound_di(FD, FI, fsz, dj, ij);
-            uint32_t eq = 0;
-            while (lb + eq < fsz && FD[lb + eq] == dj && (FI[lb + eq] & kIdMask) == ij) eq++;
-            uint32_t rs = j;
-            while (rs > 0 && Sd[rs - 1] == dj && Si[rs - 1] == ij) rs--;
-            emit = (j - rs) >= eq;
-          }
-          const uint64_t em = ballot(emit);
-          wave_sync();
-          if (emit) {
-            const uint32_t o = ne + mbcnt(em);
-            Cd[o] = dj;
-            Ci[o] = ij;
-          }
-          ne += popc(em);
-          wave_sync();
-        }
-        // ---- merge F and E, keeping the first min(beam, |F|+|E|)
+      uint32_t p_next = 0xFFFFFFFFu;  // the next expansion's position after the merge
+      if (ne > 0) {
+        // ---- merge F and E, keeping the first min(beam, |F|+|E|): emitted
+        // candidate e (its position among the emitted ones: its rank, unless
+        // a duplicate of a frontier entry was dropped) goes to e + ub_e,
+        // frontier entry i to i + #{emitted e : ub_e <= i}.  Ub[e] = ub_e is
+        // ascending in e.
         const uint32_t total = fsz + ne;
         nsz = total < beam ? total : beam;
-        float* ND = cur ? Fd0 : Fd1;
-        uint32_t* NI = cur ? Fi0 : Fi1;
-        for (uint32_t i = lane; i < fsz; i += kWave) {
-          const float fd = FD[i];
-          const uint32_t fi = FI[i];
-          const uint32_t p = i + lower_bound_di(Cd, Ci, ne, fd, fi & kIdMask);
-          if (p < nsz) { ND[p] = fd; NI[p] = fi; }
-        }
-        for (uint32_t j = lane; j < ne; j += kWave) {
-          const float ed = Cd[j];
-          const uint32_t ei = Ci[j];
-          const uint32_t p = j + upper_bound_di(FD, FI, fsz, ed, ei);
-          if (p < nsz) { ND[p] = ed; NI[p] = ei; }
+        uint2* NF = cur ? F0 : F1;
+        for (uint32_t cb = 0; cb < nc; cb += kWave) {
+          const uint32_t j = cb + lane;
+          bool em = false;
+          uint32_t e = 0, ub = 0;
+          if (j < nc) {
+            const uint2 aj = Ax[j];
+            em = !(aj.x & kFlag);
+            e = aj.x;
+            ub = aj.y;
+            if (em && ne < nc) {
+              e = 0;
+              for (uint32_t t = 0; t < nc; t++) {
+                const uint32_t rt = Ax[t].x;
+                e += (!(rt & kFlag) && rt < aj.x) ? 1u : 0u;
+              }
+            }
+          }
+          if (em) {
+            Ub[e] = ub;
+            if (e + ub < nsz) NF[e + ub] = Cn[j];
+          }
         }
         wave_sync();
+        for (uint32_t i = lane; i < fsz; i += kWave) {
+          const uint32_t p = i + count_not_above(Ub, ne, i);
+          if (p < nsz) NF[p] = FR[i];
+        }
+        // every emitted candidate is above the first unvisited entry or
+        // below it; the smallest emitted one (e = 0) lands at ub_0
+        const uint32_t ub0 = Ub[0];
+        p_next = (ui >= 0 && (uint32_t)ui < ub0) ? (uint32_t)ui : ub0;
+        wave_sync();
         cur ^= 1u;
+      } else if (ui >= 0) {
+        p_next = (uint32_t)ui;
       }
       if constexpr (GTAB) {
         pf_table = pf_id != 0xFFFFFFFFu;
         if (pf_table) {
           wave_sync_global();  // this expansion's table stores by other lanes
-          const uint32_t ps = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
-          pf_prev = lane < a.R ? table[ps] : 0xFFFFFFFFu;
+          pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
+          pf_prev = lane < a.R ? table[pf_slot] : 0xFFFFFFFFu;
         }
       }
-      float* FD2 = cur ? Fd1 : Fd0;
-      uint32_t* FI2 = cur ? Fi1 : Fi0;
-      // ---- cut trim (beamSearch.h:162-167)
-      if (a.k > 0 && nsz > a.k && a.cut_trim) {
-        const float thr = (float)(a.cut * (double)FD2[a.k]);
-        uint32_t cut_at = nsz;
-        for (uint32_t b0 = 0; b0 < nsz; b0 += kWave) {
-          const uint32_t i = b0 + lane;
-          bool over = false;
-          if (i < nsz) {
-            const float e = FD2[i];
-            over = (thr < e) || (thr == e && (FI2[i] & kIdMask) > 0u);
+      // ---- size limit and cut trim (beamSearch.h:157-167); an unchanged
+      // frontier that went through them once stays as it is
+      if (ne > 0 || !trimmed) {
+        const uint2* F2 = cur ? F1 : F0;
+        if (nsz > beam) nsz = beam;
+        if (a.k > 0 && nsz > a.k && a.cut_trim) {
+          const float thr = (float)(a.cut * (double)__uint_as_float(F2[a.k].x));
+          uint32_t cut_at = nsz;
+          for (uint32_t b0 = 0; b0 < nsz; b0 += kWave) {
+            const uint32_t i = b0 + lane;
+            bool over = false;
+            if (i < nsz) {
+              const uint2 e = F2[i];
+              const float ed = __uint_as_float(e.x);
+              over = (thr < ed) || (thr == ed && (e.y & kIdMask) > 0u);
+            }
+            const uint64_t om = ballot(over);
+            if (om) {
+              cut_at = b0 + (uint32_t)first_lane(om);
+              break;
+            }
           }
-          const uint64_t om = ballot(over);
-          if (om) { cut_at = b0 + (uint32_t)first_lane(om); break; }
+          nsz = cut_at;
         }
-        nsz = cut_at;
+        trimmed = true;
       }
       fsz = nsz;
       // ---- next unvisited frontier entry (set_difference, beamSearch.h:175-178)
-      pos = -1;
-      for (uint32_t b0 = 0; b0 < fsz; b0 += kWave) {
-        const uint32_t i = b0 + lane;
-        const bool u = i < fsz && !(FI2[i] & kFlag);
-        const uint64_t um = ballot(u);
-        if (um) { pos = (int)(b0 + (uint32_t)first_lane(um)); break; }
-      }
+      pos = p_next < fsz ? (int)p_next : -1;
     }
 
     if constexpr (TRAV != TRAV_NATIVE) {
       // FP32 re-rank of the first nr beam entries against the full rows, in
       // the reference's arithmetic, then the first k by (distance, id)
-      const uint32_t* FI = cur ? Fi1 : Fi0;
-      float* XD = cur ? Fd0 : Fd1;
-      uint32_t* XI = cur ? Fi0 : Fi1;
+      const uint2* FR = cur ? F1 : F0;
+      uint2* XR = cur ? F0 : F1;
       const uint32_t nr = (a.rerank == 0 || a.rerank > fsz) ? fsz : a.rerank;
       for (uint32_t j = lane; j < nr; j += kWave) {
-        const uint32_t id = FI[j] & kIdMask;
-        XD[j] = Dist<float, METRIC>::dist((const float*)a.base + (uint64_t)id * a.ad, (const float*)ql,
-                                          typename Dist<float, METRIC>::Q{0}, a.d, 0u);
-        XI[j] = id;
+        const uint32_t id = FR[j].y & kIdMask;
+        const float dd = Dist<float, METRIC>::dist((const float*)a.base + (uint64_t)id * a.ad, (const float*)ql,
+                                                   typename Dist<float, METRIC>::Q{0}, a.d, 0u);
+        XR[j] = make_uint2(__float_as_uint(dd), id);
       }
       wave_sync();
       for (uint32_t j = lane; j < nr; j += kWave) {
-        const float dj = XD[j];
-        const uint32_t ij = XI[j];
+        const uint2 xj = XR[j];
         uint32_t rank = 0;
         for (uint32_t t = 0; t < nr; t++) {
-          const float dt = XD[t];
-          const uint32_t it = XI[t];
-          rank += (less_di(dt, it, dj, ij) || (dt == dj && it == ij && t < j)) ? 1u : 0u;
+          const uint2 xt = XR[t];
+          rank += (less_e(xt, xj) || (same_e(xt, xj) && t < j)) ? 1u : 0u;
         }
         if (rank < a.k) {
-          a.out_ids[(uint64_t)q * a.k + rank] = ij;
-          if (a.out_dists) a.out_dists[(uint64_t)q * a.k + rank] = dj;
+          a.out_ids[(uint64_t)q * a.k + rank] = xj.y;
+          if (a.out_dists) a.out_dists[(uint64_t)q * a.k + rank] = __uint_as_float(xj.x);
         }
       }
       for (uint32_t j = nr + lane; j < a.k; j += kWave) {
@@ -443,12 +489,12 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         if (a.out_dists) a.out_dists[(uint64_t)q * a.k + j] = __builtin_inff();
       }
     } else if (!a.build_mode) {
-      const float* FD = cur ? Fd1 : Fd0;
-      const uint32_t* FI = cur ? Fi1 : Fi0;
+      const uint2* FR = cur ? F1 : F0;
       for (uint32_t j = lane; j < a.k; j += kWave) {
         const bool ok = j < fsz;
-        a.out_ids[(uint64_t)q * a.k + j] = ok ? (FI[j] & kIdMask) : 0xFFFFFFFFu;
-        if (a.out_dists) a.out_dists[(uint64_t)q * a.k + j] = ok ? FD[j] : __builtin_inff();
+        const uint2 e = ok ? FR[j] : make_uint2(0u, 0u);
+        a.out_ids[(uint64_t)q * a.k + j] = ok ? (e.y & kIdMask) : 0xFFFFFFFFu;
+        if (a.out_dists) a.out_dists[(uint64_t)q * a.k + j] = ok ? __uint_as_float(e.x) : __builtin_inff();
       }
     }
     if (lane == 0) {

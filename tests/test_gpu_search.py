@@ -176,6 +176,33 @@ def test_search_duplicate_edges(oracle, xs):
         assert np.array_equal(ids, oi) and np.array_equal(cmps, oc) and np.array_equal(vis, ov)
 
 
+def test_search_wide_rows(oracle):
+    # R = 96: more than one wave of neighbors and candidates per expansion,
+    # with duplicate edges, against the oracle at a small and a large beam
+    from distributed_diskann_amd import Index
+    from distributed_diskann_amd.formats import align_rows
+    print("[gpu] R=96 rows", flush=True)
+    rng = np.random.default_rng(5)
+    n, d, R = 1500, 16, 96
+    X = align_rows(rng.standard_normal((n, d)).astype(np.float32))
+    Q = align_rows(rng.standard_normal((64, d)).astype(np.float32))
+    P = oracle.Points(X, d)
+    G = oracle.build(P, R, 128, 1.2, tie_mode=oracle.TIE_ID).rows.copy()
+    assert G[:, 0].max() > 64
+    for i in rng.integers(0, n, 400):
+        deg = G[i, 0]
+        if 2 <= deg < R:
+            G[i, 1 + deg] = G[i, 1 + rng.integers(0, deg)]
+            G[i, 0] = deg + 1
+    idx = Index(X, d, "l2", R)
+    idx.set_graph(G)
+    for Ls in (40, 150):
+        ids, dists, cmps, vis = idx.search(Q, 10, Ls, 1.35)
+        oi, od, oc, ov = oracle.search_all(P, oracle.Graph(G), Q, 10, Ls, cut=1.35)
+        assert np.array_equal(ids, oi) and np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+        assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+
+
 def test_insert_traces_reference(xs):
     print("[gpu] build-mode traces", flush=True)
     z = xs["z"]
