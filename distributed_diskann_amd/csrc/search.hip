@@ -111,8 +111,12 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     }
     stage_traversal_query<T, METRIC, TRAV>(a, ql, qsrc);
     if constexpr (GTAB) {
+      // the slice's earlier stores complete before it is cleared, and the
+      // clear before the first filter reads it
+      wave_sync_global();
       for (uint32_t i = lane; i < (mask + 1u) / 4u; i += kWave)
         ((uint4*)table)[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+      wave_sync_global();
     } else {
       for (uint32_t i = lane; i <= mask; i += kWave) table[i] = 0xFFFFFFFFu;
     }
@@ -145,13 +149,16 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
     uint64_t nvis = 0;
     int pos = fsz > 0 ? 0 : -1;
     bool trimmed = false;  // the frontier has been through one union + trim
-    // Adjacency row of the vertex predicted to be expanded next, loaded while
-    // the current expansion is merged (degree and the first 64 neighbors).
-    // The prediction (the smallest unvisited entry of frontier and new
-    // candidates) is the vertex the next expansion takes whenever the search
-    // continues.  With a global filter the table words of those neighbors are
-    // loaded ahead too: no table store happens between that load and the
-    // next expansion's filter, so they are the words the filter would read.
+    // Adjacency row (degree and first 64 neighbors) of the vertex predicted to
+    // be expanded next.  It is requested at the head of an expansion for the
+    // first unvisited frontier entry after the current one, and again after
+    // the distances when a new candidate beats that entry; the prediction
+    // (the smallest unvisited entry of frontier and new candidates) is the
+    // vertex the next expansion takes whenever the search continues.  With a
+    // global filter the table words of those neighbors are loaded ahead too,
+    // after this expansion's table stores: no table store happens between that
+    // load and the next expansion's filter, so they are the words the filter
+    // would read.
     uint32_t pf_id = 0xFFFFFFFFu, pf_deg = 0, pf_nb = 0, pf_prev = 0xFFFFFFFFu, pf_slot = 0;
     bool pf_table = false;
 
@@ -197,6 +204,13 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         }
       }
       const uint2 ue = ui >= 0 ? FR[ui] : make_uint2(0u, 0u);
+      pf_id = ui >= 0 ? (ue.y & kIdMask) : 0xFFFFFFFFu;
+      pf_table = false;
+      if (pf_id != 0xFFFFFFFFu) {
+        const uint32_t* prow = a.graph + (uint64_t)pf_id * R1;
+        pf_nb = lane < a.R ? prow[1 + lane] : 0u;
+        pf_deg = prow[0];
+      }
 
       // ---- neighbor filter (beamSearch.h:125-131)
       const uint32_t m = deg < a.degree_limit ? deg : a.degree_limit;
@@ -208,9 +222,13 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         const bool act = valid && ((int64_t)av != self_id);
         const bool pre = b0 == 0 && use_pf_table;  // slot and table word loaded ahead
         const uint32_t slot = pre ? pf_slot : (uint32_t)hash64_2((uint64_t)av) & mask;
-        if constexpr (GTAB) wave_sync_global();  // earlier table stores of other lanes
-        uint32_t prev = 0xFFFFFFFFu;
-        if (act) prev = pre ? pf_prev : table[slot];
+        // table stores of the previous 64 neighbors by other lanes (those of
+        // earlier expansions were ordered by the fence before the table
+        // words were loaded ahead; every expansion after the first has one)
+        if (GTAB && b0 > 0) wave_sync_global();
+        uint32_t tv = 0xFFFFFFFFu;  // the table word before this neighbor list
+        if (act) tv = pre ? pf_prev : table[slot];
+        uint32_t prev = tv;
         bool has_next = false;
         // Lanes that share a slot share a bucket of `bkt`.  Every active lane
         // writes its lane id into its bucket; a lane that reads back another
@@ -242,7 +260,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         }
         const bool keep = act && (prev != av);
         wave_sync();
-        if (act && !has_next) table[slot] = av;
+        if (act && !has_next && tv != av) table[slot] = av;  // the last writer of a slot; skip same-value stores
         const uint64_t km = ballot(keep);
         if (keep) Kid[nk + mbcnt(km)] = av;
         nk += popc(km);
@@ -341,11 +359,22 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       }
       // predicted next vertex: the smaller of the first unvisited entry and
       // the smallest candidate; its adjacency row is loaded during the merge
-      pf_id = (have_min && (ui < 0 || less_e(cmin, ue))) ? cmin.y : (ui >= 0 ? (ue.y & kIdMask) : 0xFFFFFFFFu);
-      if (pf_id != 0xFFFFFFFFu) {
+      const bool cand_next = have_min && (ui < 0 || less_e(cmin, ue));
+      if (cand_next) {
+        pf_id = cmin.y;
         const uint32_t* prow = a.graph + (uint64_t)pf_id * R1;
         pf_nb = lane < a.R ? prow[1 + lane] : 0u;
         pf_deg = prow[0];
+      }
+      if constexpr (GTAB) {
+        // the head's row is the prediction: its table words are loaded
+        // during the merge
+        if (!cand_next && pf_id != 0xFFFFFFFFu) {
+          wave_sync_global();  // this expansion's table stores by other lanes
+          pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
+          pf_prev = lane < a.R ? table[pf_slot] : 0xFFFFFFFFu;
+          pf_table = true;
+        }
       }
 
       // ---- multiset union (beamSearch.h:151-154): a candidate equal to e
@@ -421,11 +450,11 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
         p_next = (uint32_t)ui;
       }
       if constexpr (GTAB) {
-        pf_table = pf_id != 0xFFFFFFFFu;
-        if (pf_table) {
-          wave_sync_global();  // this expansion's table stores by other lanes
+        if (cand_next) {  // a new candidate is the prediction: its row was loaded during the merge
+          wave_sync_global();
           pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
           pf_prev = lane < a.R ? table[pf_slot] : 0xFFFFFFFFu;
+          pf_table = true;
         }
       }
       // ---- size limit and cut trim (beamSearch.h:157-167); an unchanged
