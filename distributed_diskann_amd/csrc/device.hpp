@@ -113,13 +113,36 @@ __device__ __forceinline__ uint32_t upper_bound_di(const float* d, const uint32_
 }
 
 // ------------------------------------------------------------------ distances
-// A distance loads its row in batches of kChunkBatch 16-byte chunks, all in
-// flight before the arithmetic (one memory round trip per batch).
+// A distance walks its row in batches of kChunkBatch 16-byte chunks whose
+// loads are all in flight before the arithmetic (one memory round trip per
+// batch).  Whole batches carry no per-chunk conditions; the last, partial
+// batch re-reads in-bounds chunks and its extra terms are masked out, so no
+// chunk load or term sits behind a branch.
 constexpr uint32_t kChunkBatch = 8;
 
-__device__ __forceinline__ void load_chunks(uint4 (&xs)[kChunkBatch], const uint4* r, uint32_t c0, uint32_t n) {
+// body(x, c, ok) for the chunks c = 0 .. n-1 of row r (n >= 1); ok is false
+// for the masked tail slots, whose c repeats the last chunk
+template <typename F>
+__device__ __forceinline__ void for_chunks(const uint4* r, uint32_t n, F&& body) {
+  uint32_t c0 = 0;
+  for (; c0 + kChunkBatch <= n; c0 += kChunkBatch) {
+    uint4 xs[kChunkBatch];
 #pragma unroll
-  for (uint32_t u = 0; u < kChunkBatch; u++) xs[u] = c0 + u < n ? r[c0 + u] : make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t u = 0; u < kChunkBatch; u++) xs[u] = r[c0 + u];
+#pragma unroll
+    for (uint32_t u = 0; u < kChunkBatch; u++) body(xs[u], c0 + u, true);
+  }
+  if (c0 < n) {
+    uint4 xs[kChunkBatch];
+#pragma unroll
+    for (uint32_t u = 0; u < kChunkBatch; u++) xs[u] = r[c0 + u < n ? c0 + u : n - 1u];
+#pragma unroll
+    for (uint32_t u = 0; u < kChunkBatch; u++) body(xs[u], c0 + u < n ? c0 + u : n - 1u, c0 + u < n);
+  }
+}
+
+__device__ __forceinline__ uint4 zero_unless(uint4 x, bool ok) {
+  return ok ? x : make_uint4(0u, 0u, 0u, 0u);
 }
 
 // One lane computes one distance between a row in HBM and a reference vector
@@ -150,29 +173,22 @@ struct Dist<uint8_t, METRIC> {
   }
   __device__ static float dist(const uint8_t* row, const uint8_t* q, const Q& st, uint32_t /*d*/,
                                uint32_t nchunks) {
-    const uint4* r = (const uint4*)row;
     const uint4* v = (const uint4*)q;
     uint32_t pp = 0, qp = 0;
-    for (uint32_t c0 = 0; c0 < nchunks; c0 += kChunkBatch) {
-      uint4 xs[kChunkBatch];
-      load_chunks(xs, r, c0, nchunks);
-#pragma unroll
-      for (uint32_t u = 0; u < kChunkBatch; u++) {
-        if (c0 + u >= nchunks) break;
-        const uint4 x = xs[u];
-        const uint4 y = v[c0 + u];
-        qp = __builtin_amdgcn_udot4(x.x, y.x, qp, false);
-        qp = __builtin_amdgcn_udot4(x.y, y.y, qp, false);
-        qp = __builtin_amdgcn_udot4(x.z, y.z, qp, false);
-        qp = __builtin_amdgcn_udot4(x.w, y.w, qp, false);
-        if (METRIC == L2) {
-          pp = __builtin_amdgcn_udot4(x.x, x.x, pp, false);
-          pp = __builtin_amdgcn_udot4(x.y, x.y, pp, false);
-          pp = __builtin_amdgcn_udot4(x.z, x.z, pp, false);
-          pp = __builtin_amdgcn_udot4(x.w, x.w, pp, false);
-        }
+    for_chunks((const uint4*)row, nchunks, [&](uint4 x, uint32_t c, bool ok) {
+      x = zero_unless(x, ok);
+      const uint4 y = v[c];
+      qp = __builtin_amdgcn_udot4(x.x, y.x, qp, false);
+      qp = __builtin_amdgcn_udot4(x.y, y.y, qp, false);
+      qp = __builtin_amdgcn_udot4(x.z, y.z, qp, false);
+      qp = __builtin_amdgcn_udot4(x.w, y.w, qp, false);
+      if (METRIC == L2) {
+        pp = __builtin_amdgcn_udot4(x.x, x.x, pp, false);
+        pp = __builtin_amdgcn_udot4(x.y, x.y, pp, false);
+        pp = __builtin_amdgcn_udot4(x.z, x.z, pp, false);
+        pp = __builtin_amdgcn_udot4(x.w, x.w, pp, false);
       }
-    }
+    });
     if (METRIC == L2) return (float)(int32_t)(st.qq + pp - 2u * qp);
     return -((float)(int32_t)qp);
   }
@@ -195,29 +211,22 @@ struct Dist<int8_t, METRIC> {
   }
   __device__ static float dist(const int8_t* row, const int8_t* q, const Q& st, uint32_t /*d*/,
                                uint32_t nchunks) {
-    const uint4* r = (const uint4*)row;
     const uint4* v = (const uint4*)q;
     int32_t pp = 0, qp = 0;
-    for (uint32_t c0 = 0; c0 < nchunks; c0 += kChunkBatch) {
-      uint4 xs[kChunkBatch];
-      load_chunks(xs, r, c0, nchunks);
-#pragma unroll
-      for (uint32_t u = 0; u < kChunkBatch; u++) {
-        if (c0 + u >= nchunks) break;
-        const uint4 x = xs[u];
-        const uint4 y = v[c0 + u];
-        qp = __builtin_amdgcn_sdot4((int)x.x, (int)y.x, qp, false);
-        qp = __builtin_amdgcn_sdot4((int)x.y, (int)y.y, qp, false);
-        qp = __builtin_amdgcn_sdot4((int)x.z, (int)y.z, qp, false);
-        qp = __builtin_amdgcn_sdot4((int)x.w, (int)y.w, qp, false);
-        if (METRIC == L2) {
-          pp = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, pp, false);
-          pp = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, pp, false);
-          pp = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, pp, false);
-          pp = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, pp, false);
-        }
+    for_chunks((const uint4*)row, nchunks, [&](uint4 x, uint32_t c, bool ok) {
+      x = zero_unless(x, ok);
+      const uint4 y = v[c];
+      qp = __builtin_amdgcn_sdot4((int)x.x, (int)y.x, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.y, (int)y.y, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.z, (int)y.z, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.w, (int)y.w, qp, false);
+      if (METRIC == L2) {
+        pp = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, pp, false);
+        pp = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, pp, false);
+        pp = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, pp, false);
+        pp = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, pp, false);
       }
-    }
+    });
     if (METRIC == L2) return (float)(st.qq + pp - 2 * qp);
     return -((float)qp);
   }
@@ -226,40 +235,36 @@ struct Dist<int8_t, METRIC> {
 // FP32: the reference's own order.  L2 follows NSGDist.h:113-131 (per group
 // of four ((e0^2 + e1^2) + e2^2) + e3^2 added to the running sum, then the
 // 0-3 tail elements one by one); MIPS follows mips_point.h:60-66 (sequential
-// sum, negated).
+// sum, negated).  A masked group leaves the sum as it was.
 template <int METRIC>
 struct Dist<float, METRIC> {
   struct Q { int unused; };
   __device__ static Q prep(const float*, uint32_t) { return Q{0}; }
   __device__ static float dist(const float* row, const float* q, const Q&, uint32_t d, uint32_t /*nchunks*/) {
-    const uint4* r4 = (const uint4*)row;
     const float4* q4 = (const float4*)q;
     float acc = 0.f;
     const uint32_t ng = d >> 2;  // whole groups of four, in order, then the tail
-    for (uint32_t g0 = 0; g0 < ng; g0 += kChunkBatch) {
-      uint4 xs[kChunkBatch];
-      load_chunks(xs, r4, g0, ng);
-#pragma unroll
-      for (uint32_t u = 0; u < kChunkBatch; u++) {
-        if (g0 + u >= ng) break;
-        const float4 a = make_float4(__uint_as_float(xs[u].x), __uint_as_float(xs[u].y), __uint_as_float(xs[u].z),
-                                     __uint_as_float(xs[u].w));
-        const float4 b = q4[g0 + u];
+    if (ng > 0)
+      for_chunks((const uint4*)row, ng, [&](uint4 xr, uint32_t g, bool ok) {
+        const float4 a = make_float4(__uint_as_float(xr.x), __uint_as_float(xr.y), __uint_as_float(xr.z),
+                                     __uint_as_float(xr.w));
+        const float4 b = q4[g];
+        float t;
         if (METRIC == L2) {
           float e0 = a.x - b.x, e1 = a.y - b.y, e2 = a.z - b.z, e3 = a.w - b.w;
-          float g = e0 * e0;
-          g = g + e1 * e1;
-          g = g + e2 * e2;
-          g = g + e3 * e3;
-          acc = acc + g;
+          float gs = e0 * e0;
+          gs = gs + e1 * e1;
+          gs = gs + e2 * e2;
+          gs = gs + e3 * e3;
+          t = acc + gs;
         } else {
-          acc = acc + b.x * a.x;
-          acc = acc + b.y * a.y;
-          acc = acc + b.z * a.z;
-          acc = acc + b.w * a.w;
+          t = acc + b.x * a.x;
+          t = t + b.y * a.y;
+          t = t + b.z * a.z;
+          t = t + b.w * a.w;
         }
-      }
-    }
+        acc = ok ? t : acc;
+      });
     for (uint32_t i = ng << 2; i < d; i++) {
       if (METRIC == L2) {
         float e = row[i] - q[i];

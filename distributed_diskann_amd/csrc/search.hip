@@ -70,8 +70,13 @@ __device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t 
   return lo;
 }
 
-template <typename T, int METRIC, int TRAV, bool GTAB>
-__global__ __launch_bounds__(256) void k_search(SearchArgs a) {
+// STAGE: candidate rows are gathered into LDS before their distances (a
+// launch choice for long rows; a separate instantiation, so the other
+// kernels do not carry its registers).  The native kernels are compiled for
+// at least 6 waves per SIMD (80 VGPRs): measured faster than the compiler's
+// own choice, which hoists every staged query chunk of a distance.
+template <typename T, int METRIC, int TRAV, bool GTAB, bool STAGE>
+__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using P = TravDist<T, METRIC, TRAV>;
   const uint32_t lane = lane_id();
@@ -271,7 +276,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a) {
       // ---- distances and cutoff (beamSearch.h:135-145)
       const float cutoff = (fsz < beam) ? 2147483648.0f : __uint_as_float(FR[fsz - 1].x);
       uint32_t nc = 0;
-      if (a.stage_rows) {
+      if constexpr (STAGE) {
         // Long rows: the wave gathers stage_rows candidate rows into LDS
         // together (all lanes' loads in flight at once), then lane j computes
         // row j's distance from LDS with the same arithmetic.
@@ -553,15 +558,18 @@ __global__ __launch_bounds__(256) void k_trav_dist(SearchArgs a, const uint32_t*
   }
 }
 
-template <typename T, int M, int TR, bool G>
+template <typename T, int M, int TR, bool G, bool S>
 static hipError_t launch_g(const SearchArgs& a, const Launch& l, hipStream_t s) {
-  hipLaunchKernelGGL((k_search<T, M, TR, G>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
+  hipLaunchKernelGGL((k_search<T, M, TR, G, S>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s,
+                     a);
   return hipGetLastError();
 }
 
 template <typename T, int M, int TR>
 static hipError_t launch_t(const SearchArgs& a, const Launch& l, hipStream_t s) {
-  return a.gtable ? launch_g<T, M, TR, true>(a, l, s) : launch_g<T, M, TR, false>(a, l, s);
+  if (a.stage_rows)
+    return a.gtable ? launch_g<T, M, TR, true, true>(a, l, s) : launch_g<T, M, TR, false, true>(a, l, s);
+  return a.gtable ? launch_g<T, M, TR, true, false>(a, l, s) : launch_g<T, M, TR, false, false>(a, l, s);
 }
 
 template <typename T, int M, int TR>
@@ -574,8 +582,11 @@ static hipError_t launch_d(const SearchArgs& a, const uint32_t* ids, uint32_t m,
 
 template <typename T, int M, int TR>
 static void set_attrs_t(int lds) {
-  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, false>, at, lds);
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, false>, at, lds);
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, true>, at, lds);
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, true>, at, lds);
   (void)hipFuncSetAttribute((const void*)&k_trav_dist<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 

@@ -75,26 +75,24 @@ struct TravDist<float, METRIC, TRAV_BF16> {
   __device__ static float dist_row(const SearchArgs& a, const uint4* r, const unsigned char* ql, const Q&) {
     const float4* q = (const float4*)ql;
     float s0 = 0.f, s1 = 0.f;
-    for (uint32_t c0 = 0; c0 < a.trav_chunks; c0 += kChunkBatch) {
-      uint4 xs[kChunkBatch];
-      load_chunks(xs, r, c0, a.trav_chunks);
-#pragma unroll
-      for (uint32_t u = 0; u < kChunkBatch; u++) {
-        if (c0 + u >= a.trav_chunks) break;
-        const uint32_t c = c0 + u;
-        const uint4 x = xs[u];
-        const float4 qa = q[2 * c];
-        const float4 qb = q[2 * c + 1];
-        acc_term<METRIC>(s0, qa.x, bf16_lo(x.x));
-        acc_term<METRIC>(s1, qa.y, bf16_hi(x.x));
-        acc_term<METRIC>(s0, qa.z, bf16_lo(x.y));
-        acc_term<METRIC>(s1, qa.w, bf16_hi(x.y));
-        acc_term<METRIC>(s0, qb.x, bf16_lo(x.z));
-        acc_term<METRIC>(s1, qb.y, bf16_hi(x.z));
-        acc_term<METRIC>(s0, qb.z, bf16_lo(x.w));
-        acc_term<METRIC>(s1, qb.w, bf16_hi(x.w));
-      }
-    }
+    for_chunks(r, a.trav_chunks, [&](uint4 x, uint32_t c, bool ok) {
+      const float4 qa = q[2 * c];
+      const float4 qb = q[2 * c + 1];
+      float t0 = s0, t1 = s1;
+      acc_term<METRIC>(t0, qa.x, bf16_lo(x.x));
+      acc_term<METRIC>(t1, qa.y, bf16_hi(x.x));
+      acc_term<METRIC>(t0, qa.z, bf16_lo(x.y));
+      acc_term<METRIC>(t1, qa.w, bf16_hi(x.y));
+      acc_term<METRIC>(t0, qb.x, bf16_lo(x.z));
+      acc_term<METRIC>(t1, qb.y, bf16_hi(x.z));
+      acc_term<METRIC>(t0, qb.z, bf16_lo(x.w));
+      acc_term<METRIC>(t1, qb.w, bf16_hi(x.w));
+      s0 = ok ? t0 : s0;
+      s1 = ok ? t1 : s1;
+      // one chunk's staged query words at a time: hoisting them all costs
+      // more occupancy than their LDS latency
+      __builtin_amdgcn_sched_barrier(0);
+    });
     const float s = s0 + s1;
     return METRIC == L2 ? s : -s;
   }
@@ -136,20 +134,18 @@ struct TravDist<float, METRIC, TRAV_FP8> {
     const float4* q = (const float4*)ql;
     const float4* sc = (const float4*)(ql + a.off_qaux);
     float s0 = 0.f, s1 = 0.f;
-    for (uint32_t c0 = 0; c0 < a.trav_chunks; c0 += kChunkBatch) {
-      uint4 xs[kChunkBatch];
-      load_chunks(xs, r, c0, a.trav_chunks);
-#pragma unroll
-      for (uint32_t u = 0; u < kChunkBatch; u++) {
-        if (c0 + u >= a.trav_chunks) break;
-        const uint32_t c = c0 + u;
-        const uint4 x = xs[u];
-        fp8_word<METRIC>(s0, s1, x.x, q[4 * c + 0], sc[4 * c + 0]);
-        fp8_word<METRIC>(s0, s1, x.y, q[4 * c + 1], sc[4 * c + 1]);
-        fp8_word<METRIC>(s0, s1, x.z, q[4 * c + 2], sc[4 * c + 2]);
-        fp8_word<METRIC>(s0, s1, x.w, q[4 * c + 3], sc[4 * c + 3]);
-      }
-    }
+    for_chunks(r, a.trav_chunks, [&](uint4 x, uint32_t c, bool ok) {
+      float t0 = s0, t1 = s1;
+      fp8_word<METRIC>(t0, t1, x.x, q[4 * c + 0], sc[4 * c + 0]);
+      fp8_word<METRIC>(t0, t1, x.y, q[4 * c + 1], sc[4 * c + 1]);
+      fp8_word<METRIC>(t0, t1, x.z, q[4 * c + 2], sc[4 * c + 2]);
+      fp8_word<METRIC>(t0, t1, x.w, q[4 * c + 3], sc[4 * c + 3]);
+      s0 = ok ? t0 : s0;
+      s1 = ok ? t1 : s1;
+      // one chunk's staged query words at a time: hoisting them all costs
+      // more occupancy than their LDS latency
+      __builtin_amdgcn_sched_barrier(0);
+    });
     const float s = s0 + s1;
     return METRIC == L2 ? s : -s;
   }
