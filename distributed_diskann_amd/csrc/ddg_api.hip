@@ -147,6 +147,8 @@ static uint32_t trav_stride_for(const ddg_index* idx, int trav) {
 // Fills the traversal fields of `a` and the query staging sizes; returns the
 // per-wave LDS bytes of the staged query (plus its FP8 factors).
 static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
+  a.trav_split = 1;
+  a.trav_cps = 0;
   if (trav == DDG_TRAV_NATIVE) {
     a.qrow_bytes = round_up(idx->ad * idx->es, 16);
     a.off_qaux = a.qrow_bytes;
@@ -155,6 +157,11 @@ static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
   const uint32_t esz = trav_esize(idx, trav);
   a.trav_chunks = (idx->d * esz + 15) / 16;
   a.trav_stride = trav_stride_for(idx, trav);
+  // a reduced row is split over the fewest lanes (1, 2, 4 or 8) that leave
+  // each lane at most one load batch of 8 chunks: one memory round trip per
+  // distance
+  while (a.trav_split < 8 && (a.trav_chunks + a.trav_split - 1) / a.trav_split > 8) a.trav_split *= 2;
+  a.trav_cps = (a.trav_chunks + a.trav_split - 1) / a.trav_split;
   a.trav_rows = idx->trav_rows[trav];
   a.trav_scale = idx->trav_scale;
   const uint32_t qn = std::max<uint32_t>(idx->ad, a.trav_chunks * (16 / esz));

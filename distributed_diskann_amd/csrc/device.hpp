@@ -120,11 +120,11 @@ __device__ __forceinline__ uint32_t upper_bound_di(const float* d, const uint32_
 // chunk load or term sits behind a branch.
 constexpr uint32_t kChunkBatch = 8;
 
-// body(x, c, ok) for the chunks c = 0 .. n-1 of row r (n >= 1); ok is false
+// body(x, c, ok) for the chunks c = cb .. n-1 of row r (cb < n); ok is false
 // for the masked tail slots, whose c repeats the last chunk
 template <typename F>
-__device__ __forceinline__ void for_chunks(const uint4* r, uint32_t n, F&& body) {
-  uint32_t c0 = 0;
+__device__ __forceinline__ void for_chunks(const uint4* r, uint32_t n, F&& body, uint32_t cb = 0) {
+  uint32_t c0 = cb;
   for (; c0 + kChunkBatch <= n; c0 += kChunkBatch) {
     uint4 xs[kChunkBatch];
 #pragma unroll

@@ -35,6 +35,7 @@ struct SearchArgs {
   const void* trav_rows;
   const float* trav_scale;
   uint32_t trav_stride, trav_chunks, rerank;
+  uint32_t trav_split, trav_cps;  // lanes per reduced row and chunks per lane (split_dist)
   // visited filter in global memory (null: in LDS), gtable_words per
   // resident wave, indexed by block * waves_per_block + wave
   uint32_t* gtable;

@@ -318,6 +318,21 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           nc += popc(pm);
           wave_sync();  // the next batch overwrites the staged rows
         }
+      } else if constexpr (TRAV != TRAV_NATIVE) {
+        // a reduced row is split over trav_split lanes (one load batch each)
+        const uint32_t S = a.trav_split;
+        const uint32_t sub = lane & (S - 1u);
+        const uint32_t step = kWave / S;
+        for (uint32_t b0 = 0; b0 < nk; b0 += step) {
+          const uint32_t j = b0 + lane / S;
+          const bool v = j < nk;
+          const uint32_t id = v ? Kid[j] : 0u;
+          const float dd = split_dist<T, METRIC, TRAV>(a, id, v, ql, sub);
+          const bool pass = v && sub == 0u && !(dd >= cutoff);
+          const uint64_t pm = ballot(pass);
+          if (pass) Cn[nc + mbcnt(pm)] = make_uint2(__float_as_uint(dd), id);
+          nc += popc(pm);
+        }
       } else {
         for (uint32_t b0 = 0; b0 < nk; b0 += kWave) {
           const uint32_t j = b0 + lane;

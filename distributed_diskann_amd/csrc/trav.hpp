@@ -49,6 +49,27 @@ struct TravDist<T, METRIC, TRAV_NATIVE> {
   }
 };
 
+// The sum of a reduced row as split_dist forms it: the partial sums of its
+// a.trav_split blocks of a.trav_cps chunks, added pairwise (block 0 + 1,
+// 2 + 3, then those pairs, ...), so one lane computing a whole row returns
+// the same bits as a group of lanes splitting it.
+template <typename P>
+__device__ __forceinline__ float tree_sum(const SearchArgs& a, const uint4* r, const unsigned char* ql) {
+  float p[8];
+#pragma unroll
+  for (uint32_t k = 0; k < 8; k++) {
+    const uint32_t cb = k * a.trav_cps;
+    const uint32_t ce = cb + a.trav_cps < a.trav_chunks ? cb + a.trav_cps : a.trav_chunks;
+    p[k] = (k < a.trav_split && cb < ce) ? P::part(a, r, ql, cb, ce) : 0.f;
+  }
+#pragma unroll
+  for (uint32_t w = 1; w < 8; w <<= 1)
+#pragma unroll
+    for (uint32_t k = 0; k + w < 8; k += 2 * w)
+      if (w < a.trav_split) p[k] = p[k] + p[k + w];
+  return p[0];
+}
+
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
@@ -73,9 +94,15 @@ struct TravDist<float, METRIC, TRAV_BF16> {
     return dist_row(a, row_ptr(a, id), ql, qs);
   }
   __device__ static float dist_row(const SearchArgs& a, const uint4* r, const unsigned char* ql, const Q&) {
+    return finish(tree_sum<TravDist>(a, r, ql));
+  }
+  __device__ static float finish(float s) { return METRIC == L2 ? s : -s; }
+  // sum of the terms of chunks [cb, ce) of row r (cb < ce)
+  __device__ static float part(const SearchArgs& a, const uint4* r, const unsigned char* ql, uint32_t cb,
+                               uint32_t ce) {
     const float4* q = (const float4*)ql;
     float s0 = 0.f, s1 = 0.f;
-    for_chunks(r, a.trav_chunks, [&](uint4 x, uint32_t c, bool ok) {
+    for_chunks(r, ce, [&](uint4 x, uint32_t c, bool ok) {
       const float4 qa = q[2 * c];
       const float4 qb = q[2 * c + 1];
       float t0 = s0, t1 = s1;
@@ -92,9 +119,8 @@ struct TravDist<float, METRIC, TRAV_BF16> {
       // one chunk's staged query words at a time: hoisting them all costs
       // more occupancy than their LDS latency
       __builtin_amdgcn_sched_barrier(0);
-    });
-    const float s = s0 + s1;
-    return METRIC == L2 ? s : -s;
+    }, cb);
+    return s0 + s1;
   }
 };
 
@@ -131,10 +157,16 @@ struct TravDist<float, METRIC, TRAV_FP8> {
     return dist_row(a, row_ptr(a, id), ql, qs);
   }
   __device__ static float dist_row(const SearchArgs& a, const uint4* r, const unsigned char* ql, const Q&) {
+    return finish(tree_sum<TravDist>(a, r, ql));
+  }
+  __device__ static float finish(float s) { return METRIC == L2 ? s : -s; }
+  // sum of the terms of chunks [cb, ce) of row r (cb < ce)
+  __device__ static float part(const SearchArgs& a, const uint4* r, const unsigned char* ql, uint32_t cb,
+                               uint32_t ce) {
     const float4* q = (const float4*)ql;
     const float4* sc = (const float4*)(ql + a.off_qaux);
     float s0 = 0.f, s1 = 0.f;
-    for_chunks(r, a.trav_chunks, [&](uint4 x, uint32_t c, bool ok) {
+    for_chunks(r, ce, [&](uint4 x, uint32_t c, bool ok) {
       float t0 = s0, t1 = s1;
       fp8_word<METRIC>(t0, t1, x.x, q[4 * c + 0], sc[4 * c + 0]);
       fp8_word<METRIC>(t0, t1, x.y, q[4 * c + 1], sc[4 * c + 1]);
@@ -145,11 +177,28 @@ struct TravDist<float, METRIC, TRAV_FP8> {
       // one chunk's staged query words at a time: hoisting them all costs
       // more occupancy than their LDS latency
       __builtin_amdgcn_sched_barrier(0);
-    });
-    const float s = s0 + s1;
-    return METRIC == L2 ? s : -s;
+    }, cb);
+    return s0 + s1;
   }
 };
+
+// Distance of the row of `id` split over groups of a.trav_split lanes: lane
+// `sub` of a group sums the terms of chunks [sub * cps, (sub + 1) * cps) of
+// the reduced row (cps = a.trav_cps chunks, one load batch), and the group's
+// partial sums are added in a fixed butterfly order, so every lane of the
+// group returns the same value.  All lanes of the wave must call it; `valid`
+// lanes contribute their part, the others zero.
+template <typename T, int METRIC, int TRAV>
+__device__ __forceinline__ float split_dist(const SearchArgs& a, uint32_t id, bool valid, const unsigned char* ql,
+                                            uint32_t sub) {
+  using P = TravDist<T, METRIC, TRAV>;
+  const uint32_t cb = sub * a.trav_cps;
+  const uint32_t ce = cb + a.trav_cps < a.trav_chunks ? cb + a.trav_cps : a.trav_chunks;
+  float s = 0.f;
+  if (valid && cb < ce) s = P::part(a, P::row_ptr(a, id), ql, cb, ce);
+  for (uint32_t o = 1; o < a.trav_split; o <<= 1) s += __shfl_xor(s, (int)o);
+  return P::finish(s);
+}
 
 // Stages one query row into LDS: the first d_bytes bytes of src (whole
 // 16-byte chunks are read; a row is a multiple of 16 bytes long), zeros up to
