@@ -988,17 +988,67 @@ ddg_status ddg_build(ddg_index* idx, const ddg_build_params* bp, uint32_t* visit
   return build_common(idx, bp, visited_counts, build_seconds);
 }
 
+// The insertion order batch_insert draws for random_order (index.h:237-241):
+// parlay::random_permutation<int>(m) with the default generator, i.e. the
+// numerical-recipes hash (parlay/utilities.h:131-140) as the generator
+// (random.h:33-45); below 8192 elements one Fisher-Yates pass from the back
+// (random.h:52-60), above it a stable bucket sort of the positions by
+// hash(i) & (2^bits - 1) with bits = (ceil(log2 m) - 7) / 2 (m < 2^27) or
+// ceil(log2 m) - 17, each bucket then shuffled with its own forked state
+// (random.h:62-98).
+static uint64_t nr_hash(uint64_t u) {
+  uint64_t v = u * 3935559000370003845ull + 2691343689449507681ull;
+  v ^= v >> 21;
+  v ^= v << 37;
+  v ^= v >> 4;
+  v *= 4768777513237032717ull;
+  v ^= v << 20;
+  v ^= v >> 41;
+  v ^= v << 5;
+  return v;
+}
+
+static void fisher_yates(uint32_t* a, uint64_t n, uint64_t state) {
+  for (uint64_t i = n; i-- > 1;) std::swap(a[i], a[nr_hash(i + state) % (i + 1)]);
+}
+
+static void insertion_permutation(uint64_t m, std::vector<uint32_t>& perm) {
+  perm.resize(m);
+  if (m < 8192) {
+    for (uint64_t i = 0; i < m; i++) perm[i] = (uint32_t)i;
+    fisher_yates(perm.data(), m, 0);
+    return;
+  }
+  uint32_t lg = 0;
+  for (uint64_t b = m - 1; b; b >>= 1) lg++;
+  const uint32_t bits = m < (1ull << 27) ? (lg - 7) / 2 : lg - 17;
+  const uint64_t nb = 1ull << bits, mask = nb - 1;
+  std::vector<uint64_t> first(nb + 1, 0);
+  for (uint64_t i = 0; i < m; i++) first[(nr_hash(i) & mask) + 1]++;
+  for (uint64_t b = 0; b < nb; b++) first[b + 1] += first[b];
+  std::vector<uint64_t> next(first.begin(), first.end() - 1);
+  for (uint64_t i = 0; i < m; i++) perm[next[nr_hash(i) & mask]++] = (uint32_t)i;
+  for (uint64_t b = 0; b < nb; b++) fisher_yates(perm.data() + first[b], first[b + 1] - first[b], nr_hash(nr_hash(b)));
+}
+
 ddg_status ddg_batch_insert(ddg_index* idx, const uint32_t* inserts, uint64_t m, double alpha, uint32_t L,
                             uint32_t start_point, int random_order, double base, double max_fraction) {
   if (!idx || (!inserts && m)) return fail(idx, DDG_EINVAL, "NULL argument");
-  if (random_order) return fail(idx, DDG_EINVAL, "random_order is not supported (deterministic order only)");
+  if (random_order && m > (uint64_t)INT32_MAX) return fail(idx, DDG_EINVAL, "random_order needs m <= INT_MAX");
   if (L == 0 || !(base > 1.0)) return fail(idx, DDG_EINVAL, "bad L/base");
   if (start_point >= idx->n) return fail(idx, DDG_ERANGE, "start point out of range");
   for (uint64_t i = 0; i < m; i++)
     if (inserts[i] >= idx->n) return fail(idx, DDG_ERANGE, "ERROR: invalid point given to batch_insert");
   for (double& p : idx->prof) p = 0;
   ENSURE(B_INSERTS, std::max<uint64_t>(m, 1) * 4);
-  HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_INSERTS), inserts, m * 4, hipMemcpyHostToDevice));
+  if (random_order) {
+    std::vector<uint32_t> perm, shuffled(m);
+    insertion_permutation(m, perm);
+    for (uint64_t i = 0; i < m; i++) shuffled[i] = inserts[perm[i]];
+    HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_INSERTS), shuffled.data(), m * 4, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_INSERTS), inserts, m * 4, hipMemcpyHostToDevice));
+  }
   return batch_insert_dev(idx, buf<uint32_t>(idx, B_INSERTS), m, alpha, L, idx->R, start_point, base, max_fraction,
                           idx->n, nullptr);
 }

@@ -81,9 +81,10 @@ class Index:
         self._chk(lib().ddg_build(self.h, C.byref(bp), _c(vc), C.byref(secs)))
         return (secs.value, vc) if visited_counts else secs.value
 
-    def batch_insert(self, inserts, alpha, L, start_point=0, base=2.0, max_fraction=0.02):
+    def batch_insert(self, inserts, alpha, L, start_point=0, base=2.0, max_fraction=0.02, random_order=False):
         ins = np.ascontiguousarray(np.asarray(inserts, _U32))
-        self._chk(lib().ddg_batch_insert(self.h, _c(ins), len(ins), alpha, L, start_point, 0, base, max_fraction))
+        self._chk(lib().ddg_batch_insert(self.h, _c(ins), len(ins), alpha, L, start_point, int(bool(random_order)),
+                                         base, max_fraction))
 
     def sort_rows(self):
         self._chk(lib().ddg_sort_rows(self.h))
@@ -304,9 +305,23 @@ class knn_index:
     def __init__(self, BP: BuildParams):
         self.BP = BP
         self.start_point = 0
+        self.delete_set = set()  # index.h:52; nothing on the build or search path reads it
 
     def get_start(self):
         return self.start_point
+
+    def lazy_delete(self, deletes, G: Graph):
+        """index.h:139-164: marks points deleted.  An id above G.size() is an
+        error (the reference aborts; its bound lets G.size() through); the
+        start point is refused with a message."""
+        for p in ([deletes] if np.isscalar(deletes) else deletes):
+            p = int(p)
+            if p > G.size():
+                raise DdgError(1, f"ERROR: invalid point {p} given to lazy_delete")
+            if p == self.start_point:
+                print("Deleting start_point not permitted; continuing")
+                continue
+            self.delete_set.add(p)
 
     def build_index(self, G: Graph, Points: PointRange, BuildStats: stats | None = None):
         """index.h:125-137: start point 0, all points, final per-row sort."""
@@ -322,12 +337,10 @@ class knn_index:
 
     def batch_insert(self, inserts, G: Graph, Points: PointRange, BuildStats=None, alpha=1.2, random_order=False,
                      base=2.0, max_fraction=0.02, print_=True):
-        """index.h:213-341."""
-        if random_order:
-            raise DdgError(1, "random_order is not supported")
+        """index.h:213-341 (random_order: parlay::random_permutation's order)."""
         idx = _device_index(Points, G)
         idx.set_graph(G.rows)
-        idx.batch_insert(inserts, alpha, self.BP.L, self.start_point, base, max_fraction)
+        idx.batch_insert(inserts, alpha, self.BP.L, self.start_point, base, max_fraction, random_order)
         G.rows[...] = idx.get_graph()
 
     def robustPrune(self, p, cand, G: Graph, Points: PointRange, alpha, add=True):

@@ -99,7 +99,9 @@ ddg_status ddg_graph_get(ddg_index* idx, uint32_t* rows /* n*(R+1) */);
 ddg_status ddg_build(ddg_index* idx, const ddg_build_params* bp, uint32_t* visited_counts,
                      double* build_seconds);
 
-/* knn_index::batch_insert (index.h:213-341), random_order = 0. */
+/* knn_index::batch_insert (index.h:213-341).  random_order != 0 inserts in
+ * the order of parlay::random_permutation<int>(m) (index.h:237-241), which is
+ * deterministic (default generator state); m must then be <= INT_MAX. */
 ddg_status ddg_batch_insert(ddg_index* idx, const uint32_t* inserts, uint64_t m, double alpha, uint32_t L,
                             uint32_t start_point, int random_order, double base, double max_fraction);
 

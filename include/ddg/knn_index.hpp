@@ -20,6 +20,7 @@
 #include <cstring>
 #include <iostream>
 #include <memory>
+#include <set>
 #include <utility>
 #include <vector>
 
@@ -157,8 +158,28 @@ template <class T, class Metric = Euclidian, class indexType = uint32_t>
 struct knn_index {
   BuildParams BP;
   indexType start_point = 0;
+  std::set<indexType> delete_set;  // index.h:52; nothing on the build or search path reads it
   explicit knn_index(BuildParams& bp) : BP(bp) {}
   indexType get_start() { return start_point; }
+
+  // index.h:139-164: marks points deleted.  An id above G.size() aborts (the
+  // reference's own bound, which lets G.size() through); the start point is
+  // refused with a message.
+  void lazy_delete(indexType p, Graph& G) {
+    if (p > (long)G.size()) {
+      std::cout << "ERROR: invalid point " << p << " given to lazy_delete" << std::endl;
+      abort();
+    }
+    if (p == start_point) {
+      std::cout << "Deleting start_point not permitted; continuing" << std::endl;
+      return;
+    }
+    delete_set.insert(p);
+  }
+  template <typename Seq>
+  void lazy_delete(const Seq& deletes, Graph& G) {
+    for (indexType p : deletes) lazy_delete(p, G);
+  }
 
   void build_index(Graph& G, PointRange<T, Metric>& Points, stats<indexType>& BuildStats) {
     start_point = 0;

@@ -29,6 +29,10 @@
 //                     bsp_index.hpp:162-166,238-246  max_batch strides) at n=1e4..1e7
 //   group_by          parlay group_by_key         vs oracle/_ref (real parlay
 //                     (group_by.h:155-163)        group_by_key) on random pairs
+//   random_permutation parlay/random.h:89-159,    vs oracle/_ref (real parlay
+//                     utilities.h:131-140         random_permutation) at n from 0
+//                     (hash64), counting_sort.h   to 1M, both sides of the
+//                     (stable bucket sort)        sequential threshold
 //   batch_insert /    index.h:125-137, 213-341;   recall on random-xs against the
 //   build             bsp_index.hpp:141-434 (P=1)  reference's own GT file; graph
 //                                                 invariants (degree <= R, no self
@@ -430,6 +434,48 @@ static int batch_insert(const Points& P, GraphView& G, const uint32_t* inserts, 
 }
 
 // index.h:133-136 — sort every row by distance to its owner.
+// parlay::random_permutation<int>(n) with the default generator (state 0),
+// which batch_insert draws for random_order (index.h:237-241):
+// hash64 (utilities.h:131-140); random::ith_rand / fork (random.h:33-45);
+// seq_random_shuffle_ (random.h:52-60): Fisher-Yates from the back with
+// ith_rand(i) % (i + 1); random_shuffle_ (random.h:62-98): below 8192
+// elements one sequential shuffle, otherwise a stable bucket sort of the
+// positions by ith_rand(i) & (2^bits - 1), bits = (log2_up(n) - 7) / 2 below
+// 2^27 elements and log2_up(n) - 17 above, then bucket b shuffled with fork(b).
+static inline uint64_t phash64(uint64_t u) {
+  uint64_t v = u * 3935559000370003845ull + 2691343689449507681ull;
+  v ^= v >> 21;
+  v ^= v << 37;
+  v ^= v >> 4;
+  v *= 4768777513237032717ull;
+  v ^= v << 20;
+  v ^= v >> 41;
+  v ^= v << 5;
+  return v;
+}
+
+static void shuffle_seq(int32_t* a, uint64_t n, uint64_t state) {
+  for (uint64_t i = n; i-- > 1;) std::swap(a[i], a[phash64(i + state) % (i + 1)]);
+}
+
+static void random_permutation(uint64_t n, int32_t* out) {
+  if (n < 8192) {
+    for (uint64_t i = 0; i < n; i++) out[i] = (int32_t)i;
+    shuffle_seq(out, n, 0);
+    return;
+  }
+  uint64_t lg = 0;
+  for (uint64_t b = n - 1; b > 0; b >>= 1) lg++;
+  const uint64_t bits = n < (1ull << 27) ? (lg - 7) / 2 : lg - 17;
+  const uint64_t nb = 1ull << bits, mask = nb - 1;
+  std::vector<uint64_t> start(nb + 1, 0);
+  for (uint64_t i = 0; i < n; i++) start[(phash64(i) & mask) + 1]++;
+  for (uint64_t b = 0; b < nb; b++) start[b + 1] += start[b];
+  std::vector<uint64_t> pos(start.begin(), start.end() - 1);
+  for (uint64_t i = 0; i < n; i++) out[pos[phash64(i) & mask]++] = (int32_t)i;
+  for (uint64_t b = 0; b < nb; b++) shuffle_seq(out + start[b], start[b + 1] - start[b], phash64(phash64(b)));
+}
+
 static void sort_rows(const Points& P, GraphView& G, int tie_mode, int nthreads) {
   parallel_for(0, G.n, nthreads, [&](uint64_t i) {
     uint32_t* row = G.row(i);
@@ -627,6 +673,8 @@ int orc_build(const orc_points* P, orc_graph* G, const orc_build_params* bp, int
   }
   return r;
 }
+
+void orc_random_permutation(uint64_t n, int32_t* out) { orc::random_permutation(n, out); }
 
 int orc_sort_rows(const orc_points* P, orc_graph* G, int tie_mode, int nthreads) {
   orc::Points pp = mk(P);

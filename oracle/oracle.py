@@ -56,6 +56,8 @@ def lib():
         P = C.POINTER
         _lib.orc_hash64_2.restype = C.c_uint64
         _lib.orc_hash64_2.argtypes = [C.c_uint64]
+        _lib.orc_random_permutation.argtypes = [C.c_uint64, C.c_void_p]
+        _lib.orc_random_permutation.restype = None
         _lib.orc_beam_search.argtypes = [P(_Points), P(_Graph), C.c_void_p, C.c_int64, C.c_void_p, C.c_uint32,
                                          C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int64,
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
@@ -115,6 +117,13 @@ class Graph:
 
 def hash64_2(x: int) -> int:
     return int(lib().orc_hash64_2(x))
+
+
+def random_permutation(n: int) -> np.ndarray:
+    """parlay::random_permutation<int>(n) as batch_insert's random_order draws it."""
+    out = np.empty(n, np.int32)
+    lib().orc_random_permutation(n, _ptr(out) if n else None)
+    return out
 
 
 def distances(dtype, metric, A: np.ndarray, B: np.ndarray, d: int) -> np.ndarray:

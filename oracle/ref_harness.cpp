@@ -17,6 +17,8 @@
 //   insert <dtype> <metric> <base> <graph> <ids> <L> <start> <out>
 //                                              build-mode beam_search(Base[id]) traces
 //   groupby <pairs> <out>                      parlay::group_by_key
+//   perm <n> <out>                             parlay::random_permutation<int>(n)
+//                                              (batch_insert's random_order)
 // Vector files use the reference format [u32 n][u32 d][n*d T]; <ids> and
 // <pairs> are raw u32 arrays.
 
@@ -35,6 +37,7 @@
 #include "utils/point_range.h"
 #include "utils/types.h"
 #include "parlay/primitives.h"
+#include "parlay/random.h"
 #include "parlay/utilities.h"
 
 static std::vector<uint32_t> read_u32(const char* fn) {
@@ -143,6 +146,14 @@ int main(int argc, char** argv) {
       uint64_t h = parlay::hash64_2(i);
       fwrite(&h, 8, 1, out);
     }
+    fclose(out);
+    return 0;
+  }
+  if (mode == "perm") {
+    const int n = atoi(argv[2]);
+    auto p = parlay::random_permutation<int>(n);
+    FILE* out = fopen(argv[3], "wb");
+    fwrite(p.data(), 4, p.size(), out);
     fclose(out);
     return 0;
   }

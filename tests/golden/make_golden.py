@@ -19,6 +19,8 @@ from its headers by oracle/Makefile into oracle/_ref/ref_harness:
                   plus build-mode traces (k=0, L=50) of 100 points
   n0.npz          the same search outputs on the seeded 10,000 x 20 N(0,1)
                   workload (seed 0 base, seed 1 queries)
+  perm.npz        parlay::random_permutation<int>(n), batch_insert's
+                  random_order (`make_golden.py perm` writes only this one)
 """
 from __future__ import annotations
 
@@ -80,10 +82,32 @@ def search_fixture(tmp, dtype, metric, base_file, query_file, graph_file, nq, n,
     return res
 
 
+def make_perm(tmp):
+    """parlay::random_permutation (batch_insert's random_order): whole vectors
+    on both sides of the sequential threshold (8192) and of the bucket-count
+    steps, and the SHA-256 of one permutation above 2^27 elements (the other
+    bucket-count formula)."""
+    perm = {}
+    fo = os.path.join(tmp, "perm")
+    for n in (0, 1, 2, 3, 17, 1000, 8191, 8192, 8193, 20000, 65536, 100001, 300000):
+        run("perm", n, fo)
+        perm[f"n{n}"] = np.fromfile(fo, np.int32)
+    big = (1 << 27) + 5
+    run("perm", big, fo)
+    perm["big_n"] = np.array(big, np.int64)
+    perm["big_sha256"] = np.frombuffer(__import__("hashlib").sha256(np.fromfile(fo, np.int32).tobytes()).digest(),
+                                       np.uint8)
+    os.remove(fo)
+    np.savez_compressed(os.path.join(HERE, "perm.npz"), **perm)
+
+
 def main():
     if not (os.path.exists(HARNESS) and os.path.isdir(REF)):
         sys.exit("needs the reference tree and oracle/_ref/ref_harness (make -C oracle)")
     tmp = tempfile.mkdtemp()
+    if sys.argv[1:] == ["perm"]:  # only the permutation fixture
+        make_perm(tmp)
+        return
 
     # hash64_2
     run("hash", 4096, os.path.join(tmp, "hash"))
@@ -115,6 +139,8 @@ def main():
         dist[key + "_a"] = A; dist[key + "_b"] = B
         dist[key + "_d"] = np.fromfile(fo, np.float32)
     np.savez_compressed(os.path.join(HERE, "dist.npz"), **dist)
+
+    make_perm(tmp)
 
     # group_by_key
     keys = rng.integers(0, 300, 5000).astype(np.uint32)

@@ -83,6 +83,24 @@ def test_batch_insert_subset(oracle):
     assert np.array_equal(canon(idx.get_graph()), canon(G.rows))
 
 
+@pytest.mark.parametrize("m", [1800, 9000])
+def test_batch_insert_random_order(oracle, m):
+    # random_order = true inserts in parlay::random_permutation's order
+    # (index.h:237-241), below and above its sequential threshold (8192)
+    print(f"[gpu] batch_insert random_order m={m}", flush=True)
+    from distributed_diskann_amd import Index
+    n = 10000
+    Xa, Qa, P, _ = _small_case(oracle, "uint8", "l2", n, 32, seed=11)
+    ins = np.random.default_rng(3).permutation(n)[:m].astype(np.uint32)
+    perm = oracle.random_permutation(m)
+    G = oracle.Graph(np.zeros((n, 25), np.uint32))
+    oracle.batch_insert(P, G, ins[perm], 1.2, 40, base=2.0, max_fraction=0.02, start=int(ins[0]),
+                        tie_mode=oracle.TIE_ID)
+    idx = Index(Xa, 32, "l2", 24)
+    idx.batch_insert(ins, 1.2, 40, start_point=int(ins[0]), base=2.0, max_fraction=0.02, random_order=True)
+    assert np.array_equal(canon(idx.get_graph()), canon(G.rows))
+
+
 def test_robust_prune_vs_oracle(oracle):
     print("[gpu] robust_prune", flush=True)
     from distributed_diskann_amd import Index

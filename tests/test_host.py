@@ -135,3 +135,18 @@ def test_create_without_gpu_reports_device_error():
     with pytest.raises(DdgError) as e:
         Index(np.zeros((4, 16), np.float32), 16, "l2", 4)
     assert e.value.status in (5,)
+
+
+def test_lazy_delete_mirror():
+    # index.h:139-164: ids go to delete_set, the start point is refused, an id
+    # above G.size() is an error (G.size() itself passes, as in the reference)
+    from distributed_diskann_amd import DdgError
+    from distributed_diskann_amd.index import BuildParams, Graph, knn_index
+    G = Graph(8, 100)
+    I = knn_index(BuildParams(8, 16, 1.2, False))
+    I.lazy_delete([3, 7, 0, 100], G)
+    assert I.delete_set == {3, 7, 100}
+    I.lazy_delete(5, G)
+    assert 5 in I.delete_set
+    with pytest.raises(DdgError):
+        I.lazy_delete([101], G)

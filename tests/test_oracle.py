@@ -69,6 +69,51 @@ def test_group_by_golden(oracle):
         pos += ln
 
 
+def test_random_permutation_golden(oracle):
+    # parlay::random_permutation<int>(n) of the reference's own parlay, on both
+    # sides of the sequential threshold and of the bucket-count steps
+    z = load_golden("perm")
+    ns = [int(k[1:]) for k in z.files if k.startswith("n") and k[1:].isdigit()]
+    assert 8191 in ns and 8192 in ns and 300000 in ns
+    for n in ns:
+        p = oracle.random_permutation(n)
+        assert np.array_equal(p, z[f"n{n}"]), n
+        assert np.array_equal(np.sort(p), np.arange(n, dtype=np.int32))
+
+
+def test_random_permutation_large_branch(oracle):
+    # above 2^27 elements parlay switches its bucket-count formula; pinned by
+    # the SHA-256 of the reference's permutation of 2^27 + 5 elements
+    import hashlib
+    z = load_golden("perm")
+    p = oracle.random_permutation(int(z["big_n"]))
+    assert hashlib.sha256(p.tobytes()).digest() == z["big_sha256"].tobytes()
+
+
+def test_random_permutation_small_closed_form(oracle):
+    # below 8192 elements: Fisher-Yates from the back, swap(A[i], A[hash64(i) % (i+1)])
+    # with parlay's numerical-recipes hash64 (utilities.h:131-140), written out here
+    M = (1 << 64) - 1
+
+    def h(u):
+        v = (u * 3935559000370003845 + 2691343689449507681) & M
+        v ^= v >> 21
+        v = (v ^ (v << 37)) & M
+        v ^= v >> 4
+        v = (v * 4768777513237032717) & M
+        v = (v ^ (v << 20)) & M
+        v ^= v >> 41
+        v = (v ^ (v << 5)) & M
+        return v
+
+    for n in (2, 9, 257):
+        a = list(range(n))
+        for i in range(n - 1, 0, -1):
+            j = h(i) % (i + 1)
+            a[i], a[j] = a[j], a[i]
+        assert list(oracle.random_permutation(n)) == a
+
+
 def test_group_by_keeps_input_order(oracle):
     keys = [5, 3, 5, 5, 3, 9]
     vals = [10, 11, 12, 13, 14, 15]
