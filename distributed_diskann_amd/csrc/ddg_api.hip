@@ -155,8 +155,10 @@ static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
     return a.qrow_bytes;
   }
   const uint32_t esz = trav_esize(idx, trav);
-  a.trav_chunks = (idx->d * esz + 15) / 16;
+  // whole rows of the copy (zero beyond d, like the staged query): every
+  // distance walks complete 64-byte multiples, never a masked tail
   a.trav_stride = trav_stride_for(idx, trav);
+  a.trav_chunks = a.trav_stride / 16;
   // a reduced row is split over the fewest lanes (1, 2, 4 or 8) that leave
   // each lane at most one load batch of 8 chunks: one memory round trip per
   // distance

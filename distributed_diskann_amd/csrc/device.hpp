@@ -121,7 +121,9 @@ __device__ __forceinline__ uint32_t upper_bound_di(const float* d, const uint32_
 constexpr uint32_t kChunkBatch = 8;
 
 // body(x, c, ok) for the chunks c = cb .. n-1 of row r (cb < n); ok is false
-// for the masked tail slots, whose c repeats the last chunk
+// for the masked tail slots, whose c repeats the last chunk.  Whole rows of
+// the library's copies are multiples of four chunks (64-byte alignment), so
+// they never reach the masked tail.
 template <typename F>
 __device__ __forceinline__ void for_chunks(const uint4* r, uint32_t n, F&& body, uint32_t cb = 0) {
   uint32_t c0 = cb;
@@ -131,6 +133,14 @@ __device__ __forceinline__ void for_chunks(const uint4* r, uint32_t n, F&& body,
     for (uint32_t u = 0; u < kChunkBatch; u++) xs[u] = r[c0 + u];
 #pragma unroll
     for (uint32_t u = 0; u < kChunkBatch; u++) body(xs[u], c0 + u, true);
+  }
+  if (c0 + kChunkBatch / 2 <= n) {
+    uint4 xs[kChunkBatch / 2];
+#pragma unroll
+    for (uint32_t u = 0; u < kChunkBatch / 2; u++) xs[u] = r[c0 + u];
+#pragma unroll
+    for (uint32_t u = 0; u < kChunkBatch / 2; u++) body(xs[u], c0 + u, true);
+    c0 += kChunkBatch / 2;
   }
   if (c0 < n) {
     uint4 xs[kChunkBatch];
@@ -235,15 +245,18 @@ struct Dist<int8_t, METRIC> {
 // FP32: the reference's own order.  L2 follows NSGDist.h:113-131 (per group
 // of four ((e0^2 + e1^2) + e2^2) + e3^2 added to the running sum, then the
 // 0-3 tail elements one by one); MIPS follows mips_point.h:60-66 (sequential
-// sum, negated).  A masked group leaves the sum as it was.
+// sum, negated).  A masked group leaves the sum as it was.  The zero padding
+// of rows and staged query adds exact zeros, so MIPS, and L2 when d is a
+// multiple of four, walk the whole aligned row (nchunks groups, no tail).
 template <int METRIC>
 struct Dist<float, METRIC> {
   struct Q { int unused; };
   __device__ static Q prep(const float*, uint32_t) { return Q{0}; }
-  __device__ static float dist(const float* row, const float* q, const Q&, uint32_t d, uint32_t /*nchunks*/) {
+  __device__ static float dist(const float* row, const float* q, const Q&, uint32_t d, uint32_t nchunks) {
     const float4* q4 = (const float4*)q;
     float acc = 0.f;
-    const uint32_t ng = d >> 2;  // whole groups of four, in order, then the tail
+    const bool whole = METRIC == MIPS || (d & 3u) == 0u;
+    const uint32_t ng = whole ? nchunks : d >> 2;  // groups of four, in order, then the tail
     if (ng > 0)
       for_chunks((const uint4*)row, ng, [&](uint4 xr, uint32_t g, bool ok) {
         const float4 a = make_float4(__uint_as_float(xr.x), __uint_as_float(xr.y), __uint_as_float(xr.z),
@@ -265,7 +278,7 @@ struct Dist<float, METRIC> {
         }
         acc = ok ? t : acc;
       });
-    for (uint32_t i = ng << 2; i < d; i++) {
+    for (uint32_t i = whole ? d : ng << 2; i < d; i++) {
       if (METRIC == L2) {
         float e = row[i] - q[i];
         acc = acc + e * e;

@@ -517,7 +517,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
       for (uint32_t j = lane; j < nr; j += kWave) {
         const uint32_t id = FR[j].y & kIdMask;
         const float dd = Dist<float, METRIC>::dist((const float*)a.base + (uint64_t)id * a.ad, (const float*)ql,
-                                                   typename Dist<float, METRIC>::Q{0}, a.d, 0u);
+                                                   typename Dist<float, METRIC>::Q{0}, a.d, a.ad >> 2);
         XR[j] = make_uint2(__float_as_uint(dd), id);
       }
       wave_sync();

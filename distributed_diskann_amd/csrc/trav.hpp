@@ -13,9 +13,9 @@
 //                same way in the OCP and the FNUZ variants of e4m3), FP32
 //                accumulation.  MIPS stages q_j * s_j once per query; L2
 //                accumulates (q_j - s_j * e_ij)^2.
-// Reduced copies store rows of `trav_stride` bytes, zero-padded to whole
-// 16-byte chunks (`trav_chunks` per row); the staged query is zero beyond d,
-// so the padding adds nothing.
+// Reduced copies store rows of `trav_stride` bytes (`trav_chunks` 16-byte
+// chunks), zero beyond d; the staged query is zero beyond d too, so a
+// distance walks whole rows and the padding adds nothing.
 #pragma once
 
 #include "device.hpp"
