@@ -47,6 +47,7 @@ struct SearchArgs {
   // per-wave LDS layout (bytes); the staged query takes qrow_bytes at offset
   // 0, its FP8 factors another qrow_bytes at off_qaux
   uint32_t lds_per_wave, qrow_bytes, off_qaux, off_table, off_f0, off_f1, off_keep, off_cand, off_aux, off_bkt;
+  uint32_t off_wbits, wbits;  // global filter: LDS bitmap of the slots written by the current query (wbits != 0)
   uint32_t fcap, rcap;
 };
 

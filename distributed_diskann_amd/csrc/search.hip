@@ -94,6 +94,10 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
   uint2* Cn = (uint2*)(my + a.off_cand);   // candidates of one expansion, in neighbor order
   uint2* Ax = (uint2*)(my + a.off_aux);    // per candidate: (rank, entries of F not above it)
   uint32_t* Ub = Kid;
+  uint32_t* wb = (uint32_t*)(my + a.off_wbits);  // written-slot bitmap (GTAB with wbits)
+  // the bitmap is used by the native kernels only (the host sets wbits
+  // only for them); the reduced-precision kernels compile without it
+  const bool WB = GTAB && TRAV == TRAV_NATIVE && a.wbits != 0u;
 
   const uint64_t R1 = (uint64_t)a.R + 1;
   const T* base = (const T*)a.base;
@@ -116,12 +120,17 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
     }
     stage_traversal_query<T, METRIC, TRAV>(a, ql, qsrc);
     if constexpr (GTAB) {
-      // the slice's earlier stores complete before it is cleared, and the
-      // clear before the first filter reads it
+      // the slice's earlier stores complete first; then either the bitmap of
+      // written slots is cleared (the slice is only read where a bit is set,
+      // i.e. where this query stored) or the slice itself is
       wave_sync_global();
-      for (uint32_t i = lane; i < (mask + 1u) / 4u; i += kWave)
-        ((uint4*)table)[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
-      wave_sync_global();
+      if (WB) {
+        for (uint32_t i = lane; i < (mask + 1u) / 32u; i += kWave) wb[i] = 0u;
+      } else {
+        for (uint32_t i = lane; i < (mask + 1u) / 4u; i += kWave)
+          ((uint4*)table)[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        wave_sync_global();
+      }
     } else {
       for (uint32_t i = lane; i <= mask; i += kWave) table[i] = 0xFFFFFFFFu;
     }
@@ -232,7 +241,10 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         // words were loaded ahead; every expansion after the first has one)
         if (GTAB && b0 > 0) wave_sync_global();
         uint32_t tv = 0xFFFFFFFFu;  // the table word before this neighbor list
-        if (act) tv = pre ? pf_prev : table[slot];
+        if (act) {
+          if (pre) tv = pf_prev;
+          else if (!WB || ((wb[slot >> 5] >> (slot & 31u)) & 1u)) tv = table[slot];
+        }
         uint32_t prev = tv;
         bool has_next = false;
         // Lanes that share a slot share a bucket of `bkt`.  Every active lane
@@ -265,7 +277,10 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         }
         const bool keep = act && (prev != av);
         wave_sync();
-        if (act && !has_next && tv != av) table[slot] = av;  // the last writer of a slot; skip same-value stores
+        if (act && !has_next && tv != av) {  // the last writer of a slot; skip same-value stores
+          table[slot] = av;
+          if (WB) atomicOr(&wb[slot >> 5], 1u << (slot & 31u));
+        }
         const uint64_t km = ballot(keep);
         if (keep) Kid[nk + mbcnt(km)] = av;
         nk += popc(km);
@@ -392,7 +407,8 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         if (!cand_next && pf_id != 0xFFFFFFFFu) {
           wave_sync_global();  // this expansion's table stores by other lanes
           pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
-          pf_prev = lane < a.R ? table[pf_slot] : 0xFFFFFFFFu;
+          pf_prev = (lane < a.R && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u))) ? table[pf_slot]
+                                                                                         : 0xFFFFFFFFu;
           pf_table = true;
         }
       }
@@ -473,7 +489,8 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         if (cand_next) {  // a new candidate is the prediction: its row was loaded during the merge
           wave_sync_global();
           pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
-          pf_prev = lane < a.R ? table[pf_slot] : 0xFFFFFFFFu;
+          pf_prev = (lane < a.R && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u))) ? table[pf_slot]
+                                                                                         : 0xFFFFFFFFu;
           pf_table = true;
         }
       }
