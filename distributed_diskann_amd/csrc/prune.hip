@@ -1,13 +1,14 @@
 // robustPrune (index.h:63-123) — one wavefront per point, persistent launch.
 //
 // Task t prunes point p = pts[t] over candidates A_t (ids, with distances to
-// p or without), plus G[p] when `add` (index.h:70-75).  Candidates are
-// ordered by (distance, id) with a bitonic sort in LDS; the reference's
-// std::sort orders by distance only, so equal-distance candidates may come
-// out in another order there (its tie order is libstdc++'s, not a contract).
-// Then the greedy sweep of index.h:84-106: take the next candidate that is
-// neither p nor pruned, and prune every later candidate p' with
-// alpha * d(p*, p') <= d(p, p') evaluated in double.
+// p or without), plus G[p] when `add` (index.h:70-75).  The greedy sweep of
+// index.h:76-106 over candidates sorted by (distance, id) — take the next
+// candidate that is neither p nor pruned, and prune every later candidate p'
+// with alpha * d(p*, p') <= d(p, p') evaluated in double — runs without the
+// sort: each step takes the smallest (distance, id) of the live candidates.
+// The reference's std::sort orders by distance only, so equal-distance
+// candidates may come out in another order there (its tie order is
+// libstdc++'s, not a contract).
 //
 // Modes: forward (write up to Rsel ids per task to out_ids/out_len), and
 // reverse (append_if_fits: if |A| + deg(p) <= Rsel append A to G[p]
@@ -79,74 +80,105 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
       Cd[j] = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(r, pv, ps, a.d, nchunks);
       if (STAGED) Cs[j] = (uint16_t)j;
     }
-    uint32_t P2 = 1;
-    while (P2 < M) P2 <<= 1;
-    for (uint32_t j = M + lane; j < P2; j += kWave) {
-      Cd[j] = __builtin_inff();
-      Ci[j] = kPruned;
-      if (STAGED) Cs[j] = 0;
-    }
     if (BIG) __threadfence_block();
     wave_sync();
 
-    // ---- bitonic sort of P2 entries by (distance, id)
-    for (uint32_t kk = 2; kk <= P2; kk <<= 1) {
-      for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
-        for (uint32_t i = lane; i < (P2 >> 1); i += kWave) {
-          const uint32_t lo = ((i / jj) * (jj << 1)) + (i % jj);
-          const uint32_t hi = lo + jj;
-          const bool up = (lo & kk) == 0;
-          const float dl = Cd[lo], dh = Cd[hi];
-          const uint32_t il = Ci[lo], ih = Ci[hi];
-          const bool sw = up ? less_di(dh, ih, dl, il) : less_di(dl, il, dh, ih);
-          if (sw) {
-            Cd[lo] = dh; Cd[hi] = dl; Ci[lo] = ih; Ci[hi] = il;
-            if (STAGED) { const uint16_t sl = Cs[lo]; Cs[lo] = Cs[hi]; Cs[hi] = sl; }
-          }
-        }
-        if (BIG) __threadfence_block();
-        wave_sync();
+    // ---- greedy selection (index.h:76-106) without a sort.  Sorting by
+    // (distance, id) and taking the first candidate that is neither pruned
+    // nor p is the same as taking, each step, the smallest (distance, id) of
+    // the live candidates: every candidate the sorted sweep passed is either
+    // selected, pruned or p itself.  So candidates equal to p are dropped
+    // first, then each step selects the live minimum, prunes the other live
+    // candidates against it (they all come after it in sorted order; an equal
+    // copy of it too), compacts the survivors to the front in their order and
+    // finds the next minimum in the same pass.
+    float bd = __builtin_inff();
+    uint32_t bi = kPruned, bp = kPruned;  // this lane's smallest (distance, id, position)
+    auto offer = [&](float d, uint32_t i, uint32_t pos) {
+      if (d < bd || (d == bd && (i < bi || (i == bi && pos < bp)))) { bd = d; bi = i; bp = pos; }
+    };
+    auto wave_min = [&]() {
+      for (int o = 1; o < (int)kWave; o <<= 1) {
+        const float od = __shfl_xor(bd, o);
+        const uint32_t oi = __shfl_xor(bi, o), op = __shfl_xor(bp, o);
+        offer(od, oi, op);
       }
+      bp = bcast0(bp);
+      bi = bcast0(bi);
+    };
+    uint32_t live = 0;
+    for (uint32_t b0 = 0; b0 < M; b0 += kWave) {
+      const uint32_t j = b0 + lane;
+      bool keep = false;
+      uint32_t c = 0, cs = 0;
+      float cd = 0.f;
+      if (j < M) {
+        c = Ci[j];
+        cd = Cd[j];
+        if (STAGED) cs = Cs[j];
+        keep = c != p;
+      }
+      const uint64_t km = ballot(keep);
+      wave_sync();  // every lane has read its entry before any is overwritten
+      if (keep) {
+        const uint32_t o = live + mbcnt(km);  // o <= j: entries move left
+        Ci[o] = c;
+        Cd[o] = cd;
+        if (STAGED) Cs[o] = (uint16_t)cs;
+        offer(cd, c, o);
+      }
+      live += popc(km);
+      if (BIG) __threadfence_block();
+      wave_sync();
     }
+    wave_min();
 
-    // ---- greedy selection (index.h:84-106)
-    uint32_t nsel = 0, idx = 0;
-    while (nsel < a.Rsel && idx < M) {
-      int found = -1;
-      for (uint32_t b0 = idx; b0 < M; b0 += kWave) {
-        const uint32_t i = b0 + lane;
-        bool ok = false;
-        if (i < M) {
-          const uint32_t c = Ci[i];
-          ok = c != kPruned && c != p;
-        }
-        const uint64_t m = ballot(ok);
-        if (m) { found = (int)(b0 + (uint32_t)first_lane(m)); break; }
-      }
-      if (found < 0) break;
-      const uint32_t pstar = Ci[found];
+    uint32_t nsel = 0;
+    while (bp != kPruned) {
+      const uint32_t pstar = bi, at = bp;
       if (lane == 0) sel[nsel] = pstar;
       nsel++;
-      idx = (uint32_t)found + 1u;
       if (nsel >= a.Rsel) break;  // the reference's remaining prunes cannot change the result
       const T* ps_row = sv;
       if constexpr (STAGED) {
-        ps_row = rows + (uint64_t)Cs[found] * a.rstride;
+        ps_row = rows + (uint64_t)Cs[at] * a.rstride;
       } else {
         stage_row(sv, base + (uint64_t)pstar * a.ad, row_bytes);
         wave_sync();
       }
       const typename D::Q ss = D::prep(ps_row, nchunks);
-      for (uint32_t j = idx + lane; j < M; j += kWave) {
-        const uint32_t c = Ci[j];
-        if (c != kPruned) {
-          const T* r = STAGED ? rows + (uint64_t)Cs[j] * a.rstride : base + (uint64_t)c * a.ad;
+      bd = __builtin_inff();
+      bi = bp = kPruned;
+      uint32_t kept = 0;
+      for (uint32_t b0 = 0; b0 < live; b0 += kWave) {
+        const uint32_t j = b0 + lane;
+        bool keep = false;
+        uint32_t c = 0, cs = 0;
+        float cd = 0.f;
+        if (j < live && j != at) {
+          c = Ci[j];
+          cd = Cd[j];
+          if (STAGED) cs = Cs[j];
+          const T* r = STAGED ? rows + (uint64_t)cs * a.rstride : base + (uint64_t)c * a.ad;
           const float dsp = D::dist(r, ps_row, ss, a.d, nchunks);
-          if (a.alpha * (double)dsp <= (double)Cd[j]) Ci[j] = kPruned;
+          keep = !(a.alpha * (double)dsp <= (double)cd);
         }
+        const uint64_t km = ballot(keep);
+        if (BIG) __threadfence_block();
+        wave_sync();  // every lane has read its entry (and p*'s row slot) before any is overwritten
+        if (keep) {
+          const uint32_t o = kept + mbcnt(km);
+          Ci[o] = c;
+          Cd[o] = cd;
+          if (STAGED) Cs[o] = (uint16_t)cs;
+          offer(cd, c, o);
+        }
+        kept += popc(km);
+        if (BIG) __threadfence_block();
+        wave_sync();
       }
-      if (BIG) __threadfence_block();
-      wave_sync();
+      live = kept;
+      wave_min();
     }
     wave_sync();
     if (a.write_graph) {
