@@ -290,6 +290,12 @@ static uint32_t prune_ccap_max(uint32_t dflt) {
   return c ? (uint32_t)c : dflt;
 }
 constexpr uint32_t kForwardCcap = 512, kReverseCcap = 128, kAnyCcap = 4096;
+// DDG_PRUNE_GRAM=0 prunes integer rows pair by pair instead of from the
+// Gram matrix (measurement only; the same selections)
+static bool prune_gram_enabled() {
+  static const int on = env_int("DDG_PRUNE_GRAM", 1, 0, 1);
+  return on != 0;
+}
 
 // Per-wave LDS of a prune launch: p's and p*'s rows, the selection, ccap
 // candidates (distance, id, row slot), then a cache of rrows candidate rows
@@ -308,10 +314,20 @@ static void prune_layout(const ddg_index* idx, uint32_t Rsel, uint32_t want_ccap
   a.ccap = ccap;
   a.off_cand = 2 * row + round_up(Rsel * 4, 16);
   a.off_rows = fixed(ccap);
-  uint32_t rrows = fixed(ccap) < budget ? (budget - fixed(ccap)) / rs : 0u;
-  rrows = std::min<uint32_t>(rrows, std::min<uint32_t>(ccap, round_up(std::max<uint32_t>(want_ccap, 16), 16)));
-  a.rrows = rrows >= 16 ? rrows : 0u;
-  a.lds_per_wave = round_up(fixed(ccap) + a.rrows * rs, 16);
+  // integer rows of at most 256 bytes: tasks of up to 256 candidates prune
+  // from an MFMA Gram matrix (its LDS replaces the row cache)
+  a.gram_max = 0;
+  a.off_gram = fixed(ccap);
+  if (idx->es == 1 && idx->ad <= 256 && prune_gram_enabled()) {
+    a.gram_max = std::min<uint32_t>(ccap, 256);
+    a.rrows = 0;
+    a.lds_per_wave = round_up(fixed(ccap) + prune_gram_lds_bytes(), 16);
+  } else {
+    uint32_t rrows = fixed(ccap) < budget ? (budget - fixed(ccap)) / rs : 0u;
+    rrows = std::min<uint32_t>(rrows, std::min<uint32_t>(ccap, round_up(std::max<uint32_t>(want_ccap, 16), 16)));
+    a.rrows = rrows >= 16 ? rrows : 0u;
+    a.lds_per_wave = round_up(fixed(ccap) + a.rrows * rs, 16);
+  }
   // the block shape (1, 2 or 4 waves) that fits the most waves on a CU
   uint32_t wpb = 1, bpc = 1, best = 0;
   for (uint32_t w : {4u, 2u, 1u}) {

@@ -79,6 +79,9 @@ struct PruneArgs {
   // LDS row cache: tasks with at most rrows candidates stage their rows at
   // off_rows, rstride elements apart
   uint32_t rrows, off_rows, rstride;
+  // integer rows: tasks with at most gram_max candidates (<= 256) prune from
+  // an MFMA Gram matrix; its norms, sums and prune bit matrix live at off_gram
+  uint32_t gram_max, off_gram;
 };
 
 struct Launch {
@@ -102,6 +105,8 @@ hipError_t launch_prune(int dtype, int metric, const PruneArgs& a, const Launch&
 // raise the dynamic-LDS limit of every search / prune kernel (once per process)
 void search_set_attributes(int max_lds);
 void prune_set_attributes(int max_lds);
+// LDS bytes the Gram-matrix prune path needs per wave
+uint32_t prune_gram_lds_bytes();
 
 // edges.hip
 hipError_t launch_apply_rows(uint32_t* graph, uint32_t R1, const uint32_t* pts, const uint32_t* new_ids,

@@ -192,6 +192,192 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
   }
 }
 
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+constexpr uint32_t kGramMax = 256;  // candidates of a Gram-matrix task
+constexpr uint32_t kGramWords = kGramMax / 32;
+
+// LDS bytes of the Gram-matrix path: norms and sums (int32 each) and the
+// prune bit matrix (kGramMax rows of kGramMax bits)
+__host__ __device__ constexpr uint32_t gram_lds_bytes() { return kGramMax * 8u + kGramMax * kGramWords * 4u; }
+
+// robustPrune of one task over integer rows (uint8 / int8, d <= 256) from a
+// Gram matrix on the matrix cores.  Candidates are sorted by (distance, id)
+// (candidates equal to p last, never selected); for every pair a < b in that
+// order the exact integer distance d(a, b) comes from one 16 x 16 x 64
+// v_mfma_i32_16x16x64_i8 tile (|a|^2 + |b|^2 - 2 a.b for L2, -a.b for MIPS;
+// uint8 enters as x - 128, corrected by 128 (sum a + sum b) - 16384 ad), and
+// bit b of row a records whether a would prune b: alpha * d(a, b) <= d(p, b)
+// in double (index.h:97-103).  The greedy sweep (index.h:84-106) is then a
+// walk over bit rows: take the lowest live position, clear its row's bits.
+// Same integers, same float casts and the same double test as the per-pair
+// distances of prune_task, so the selections are identical.
+template <typename T, int METRIC>
+__device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, uint32_t p, uint32_t* grow,
+                                                uint64_t a0, uint64_t alen, uint32_t gdeg, T* pv, uint32_t* sel,
+                                                float* Cd, uint32_t* Ci, unsigned char* gram) {
+  using D = Dist<T, METRIC>;
+  constexpr bool U8 = sizeof(T) == 1 && T(-1) > T(0);
+  constexpr uint32_t kFlip = U8 ? 0x80808080u : 0u;
+  const uint32_t lane = lane_id();
+  const uint32_t row_bytes = a.ad * (uint32_t)sizeof(T);
+  const uint32_t nchunks = row_bytes >> 4;
+  const T* base = (const T*)a.base;
+  const uint32_t M = (uint32_t)(alen + gdeg);
+  int32_t* Nrm = (int32_t*)gram;
+  int32_t* Sm = Nrm + kGramMax;
+  uint32_t* Pm = (uint32_t*)(Sm + kGramMax);
+  stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
+  for (uint32_t j = lane; j < M; j += kWave) Ci[j] = j < alen ? a.a_ids[a0 + j] : grow[1 + (j - (uint32_t)alen)];
+  wave_sync();
+  const typename D::Q ps = D::prep(pv, nchunks);
+  // distances to p; candidates equal to p sort last and are never taken
+  uint32_t P2 = 1;
+  while (P2 < M) P2 <<= 1;
+  for (uint32_t j = lane; j < P2; j += kWave) {
+    float dj = __builtin_inff();
+    uint32_t ij = kPruned;
+    if (j < M) {
+      ij = Ci[j];
+      if (ij == p) ij = kPruned;
+      else dj = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(base + (uint64_t)ij * a.ad, pv, ps, a.d, nchunks);
+    }
+    Cd[j] = dj;
+    Ci[j] = ij;
+  }
+  wave_sync();
+  // bitonic sort by (distance, id)
+  for (uint32_t kk = 2; kk <= P2; kk <<= 1) {
+    for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
+      for (uint32_t i = lane; i < (P2 >> 1); i += kWave) {
+        const uint32_t lo = ((i / jj) * (jj << 1)) + (i % jj);
+        const uint32_t hi = lo + jj;
+        const bool up = (lo & kk) == 0;
+        const float dl = Cd[lo], dh = Cd[hi];
+        const uint32_t il = Ci[lo], ih = Ci[hi];
+        const bool sw = up ? less_di(dh, ih, dl, il) : less_di(dl, il, dh, ih);
+        if (sw) { Cd[lo] = dh; Cd[hi] = dl; Ci[lo] = ih; Ci[hi] = il; }
+      }
+      wave_sync();
+    }
+  }
+  uint32_t Mp = 0;  // candidates other than p
+  for (uint32_t b0 = 0; b0 < M; b0 += kWave) Mp += popc(ballot(b0 + lane < M && Ci[b0 + lane] != kPruned));
+  // norms and sums of the sorted candidates (sums only matter for uint8)
+  for (uint32_t j = lane; j < Mp; j += kWave) {
+    const uint4* r = (const uint4*)(base + (uint64_t)Ci[j] * a.ad);
+    int32_t nn = 0, ss = 0;
+    for (uint32_t c = 0; c < nchunks; c++) {
+      const uint4 x = r[c];
+      if constexpr (U8) {
+        nn = (int32_t)__builtin_amdgcn_udot4(x.x, x.x, (uint32_t)nn, false);
+        nn = (int32_t)__builtin_amdgcn_udot4(x.y, x.y, (uint32_t)nn, false);
+        nn = (int32_t)__builtin_amdgcn_udot4(x.z, x.z, (uint32_t)nn, false);
+        nn = (int32_t)__builtin_amdgcn_udot4(x.w, x.w, (uint32_t)nn, false);
+        ss = (int32_t)__builtin_amdgcn_udot4(x.x, 0x01010101u, (uint32_t)ss, false);
+        ss = (int32_t)__builtin_amdgcn_udot4(x.y, 0x01010101u, (uint32_t)ss, false);
+        ss = (int32_t)__builtin_amdgcn_udot4(x.z, 0x01010101u, (uint32_t)ss, false);
+        ss = (int32_t)__builtin_amdgcn_udot4(x.w, 0x01010101u, (uint32_t)ss, false);
+      } else {
+        nn = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, nn, false);
+        nn = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, nn, false);
+        nn = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, nn, false);
+        nn = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, nn, false);
+      }
+    }
+    Nrm[j] = nn;
+    Sm[j] = ss;
+  }
+  for (uint32_t i = lane; i < Mp * kGramWords; i += kWave) Pm[i] = 0u;
+  wave_sync();
+
+  // the upper triangle of the Gram matrix in 16 x 16 tiles; v_mfma_i32_16x16x64_i8
+  // leaves row 4 (lane / 16) + r, column lane % 16 in accumulator register r;
+  // lane (g4, c16) supplies 16 bytes at element g4 * Q + 16 s of its row for
+  // K step s (the same K order for both operands)
+  const uint32_t g4 = lane >> 4, c16 = lane & 15u;
+  const uint32_t Q = a.ad / 4u, nsteps = Q / 16u;
+  const uint32_t nb = (Mp + 15u) / 16u;
+  for (uint32_t I = 0; I < nb; I++) {
+    const uint32_t ra = 16u * I + c16 < Mp ? 16u * I + c16 : Mp - 1u;
+    const unsigned char* rowa = (const unsigned char*)(base + (uint64_t)Ci[ra] * a.ad);
+    v4i_t af[4];
+#pragma unroll
+    for (uint32_t st = 0; st < 4; st++) {
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (st < nsteps) v = *(const uint4*)(rowa + g4 * Q + 16u * st);
+      af[st] = v4i_t{(int)(v.x ^ kFlip), (int)(v.y ^ kFlip), (int)(v.z ^ kFlip), (int)(v.w ^ kFlip)};
+    }
+    int32_t na[4], sa[4];
+#pragma unroll
+    for (uint32_t r = 0; r < 4; r++) {
+      const uint32_t ar = 16u * I + 4u * g4 + r;
+      na[r] = ar < Mp ? Nrm[ar] : 0;
+      sa[r] = ar < Mp ? Sm[ar] : 0;
+    }
+    for (uint32_t J = I; J < nb; J++) {
+      const uint32_t rb = 16u * J + c16 < Mp ? 16u * J + c16 : Mp - 1u;
+      const unsigned char* rowb = (const unsigned char*)(base + (uint64_t)Ci[rb] * a.ad);
+      v4i_t acc = v4i_t{0, 0, 0, 0};
+#pragma unroll
+      for (uint32_t st = 0; st < 4; st++) {
+        if (st < nsteps) {
+          const uint4 v = *(const uint4*)(rowb + g4 * Q + 16u * st);
+          const v4i_t bv = v4i_t{(int)(v.x ^ kFlip), (int)(v.y ^ kFlip), (int)(v.z ^ kFlip), (int)(v.w ^ kFlip)};
+          acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[st], bv, acc, 0, 0, 0);
+        }
+      }
+      const uint32_t b = 16u * J + c16;
+      const int32_t nbv = b < Mp ? Nrm[b] : 0, sbv = b < Mp ? Sm[b] : 0;
+      const double cdb = b < Mp ? (double)Cd[b] : 0.0;
+#pragma unroll
+      for (uint32_t r = 0; r < 4; r++) {
+        const uint32_t ar = 16u * I + 4u * g4 + r;
+        int32_t dot = acc[r];
+        if (U8) dot += 128 * (sa[r] + sbv) - 16384 * (int32_t)a.ad;
+        const float dab = METRIC == L2 ? (float)(int32_t)(na[r] + nbv - 2 * dot) : -((float)dot);
+        const bool pr = ar < Mp && b < Mp && b > ar && a.alpha * (double)dab <= cdb;
+        const uint64_t m = ballot(pr);
+        if (c16 == 0) {
+          const uint32_t bits = (uint32_t)(m >> (16u * g4)) & 0xFFFFu;
+          if (bits) atomicOr(&Pm[ar * kGramWords + (J >> 1)], bits << ((J & 1u) * 16u));
+        }
+      }
+    }
+  }
+  wave_sync();
+
+  // greedy sweep over the bit rows: lane w < kGramWords holds live word w
+  uint32_t live = 0;
+  if (lane < kGramWords) {
+    const uint32_t lo = lane * 32u;
+    live = Mp >= lo + 32u ? 0xFFFFFFFFu : (Mp > lo ? (1u << (Mp - lo)) - 1u : 0u);
+  }
+  uint32_t nsel = 0;
+  for (;;) {
+    const uint64_t nz = ballot(lane < kGramWords && live != 0u);
+    if (!nz) break;
+    const uint32_t w = (uint32_t)first_lane(nz);
+    const uint32_t lw = readlane_u32(live, (int)w);
+    const uint32_t at = w * 32u + (uint32_t)__builtin_ctz(lw);
+    if (lane == 0) sel[nsel] = Ci[at];
+    nsel++;
+    if (nsel >= a.Rsel) break;
+    if (lane < kGramWords) {
+      live &= ~Pm[at * kGramWords + lane];
+      if (lane == w) live &= ~(1u << (at & 31u));
+    }
+  }
+  wave_sync();
+  if (a.write_graph) {
+    for (uint32_t j = lane; j < nsel; j += kWave) grow[1 + j] = sel[j];
+    wave_sync();
+    if (lane == 0) grow[0] = nsel;
+  } else {
+    for (uint32_t j = lane; j < nsel; j += kWave) a.out_ids[(uint64_t)t * a.Rsel + j] = sel[j];
+    if (lane == 0) a.out_len[t] = nsel;
+  }
+}
+
 template <typename T, int METRIC, bool BIG>
 __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -256,6 +442,8 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
           atomicOr(a.status, 2u);
         }
       }
+    } else if (!BIG && sizeof(T) == 1 && M64 <= a.gram_max) {
+      prune_task_gram<T, METRIC>(a, t, p, grow, a0, alen, gdeg, pv, sel, Cd, Ci, my + a.off_gram);
     } else if (!BIG && M64 <= a.rrows) {
       prune_task<T, METRIC, false, true>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
     } else {
@@ -270,6 +458,8 @@ static void set_attrs_t(int lds) {
   (void)hipFuncSetAttribute((const void*)&k_prune<T, M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipFuncSetAttribute((const void*)&k_prune<T, M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
+
+uint32_t prune_gram_lds_bytes() { return gram_lds_bytes(); }
 
 void prune_set_attributes(int max_lds) {
   set_attrs_t<uint8_t, L2>(max_lds);
