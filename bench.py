@@ -129,12 +129,14 @@ def profiled_traffic(workload, n, Ls):
     """HBM bytes per search launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
     (copy-calibrated) for this exact workload and beam, from the committed
     profile summary; None when no profile covers it."""
-    path = os.path.join(HERE, "profiles", "k_search_f32_100Kx20_Ls80.md")
-    if workload != "s" or n != 100_000 or Ls != 80 or not os.path.exists(path):
+    covered = {("s", 100_000, 80): "k_search_f32_100Kx20_Ls80.md",
+               ("u8", 10_000_000, 140): "k_search_u8_10Mx128_Ls140.md"}
+    name = covered.get((workload, n, Ls))
+    if name is None or not os.path.exists(os.path.join(HERE, "profiles", name)):
         return None, None
-    for line in open(path):
+    for line in open(os.path.join(HERE, "profiles", name)):
         if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1]), "profiles/k_search_f32_100Kx20_Ls80.md"
+            return float(line.split(":")[1]), "profiles/" + name
     return None, None
 
 
