@@ -271,6 +271,7 @@ static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nsta
 //   DDG_PRUNE_LDS=<KB>   per-wave LDS budget of robustPrune (0: no row cache)
 //   DDG_PRUNE_CCAP=<n>   candidates a prune keeps in LDS; longer lists take
 //                        the global-scratch pass
+//   DDG_PRUNE_BIG_WAVES=<w> waves sharing one task of the global-scratch pass
 static int env_int(const char* name, int dflt, int lo, int hi) {
   const char* e = getenv(name);
   if (!e) return dflt;
@@ -281,6 +282,13 @@ static int env_int(const char* name, int dflt, int lo, int hi) {
 // prunes keep up to 512 candidates in LDS (visited lists), reverse prunes
 // 128 (|S| + R for most targets; hub targets with thousands of new
 // in-edges take the global-scratch pass instead of inflating every wave).
+// Waves of a block of the queued-task pass (1, 2, 4, 8 or 16).  8 measured
+// best over the 10M x 128 uint8 and 1M x 100 int8 builds (4 suits the int8
+// build's shorter lists, 16 the uint8 build's hubs).
+static uint32_t prune_big_waves() {
+  static const int w = env_int("DDG_PRUNE_BIG_WAVES", 8, 1, (int)kMaxBigWaves);
+  return (uint32_t)(w & (w - 1) ? 8 : w);
+}
 static uint32_t prune_wave_lds() {
   static const int kb = env_int("DDG_PRUNE_LDS", 12, 0, 160);
   return (uint32_t)kb * 1024u;
@@ -492,9 +500,11 @@ static ddg_status prune_prepare(ddg_index* idx, PrunePlan& pp, uint64_t np, uint
   a.ad = idx->ad;
   a.R1 = idx->R + 1;
   pp.l.blocks = (uint32_t)std::min<uint64_t>(pp.l.blocks, (np + pp.l.waves_per_block - 1) / pp.l.waves_per_block);
-  pp.lb.waves_per_block = 4;
-  pp.lb.blocks = big_waves / 4;
-  pp.lb.lds_bytes = 4 * round_up(2 * round_up(idx->ad * idx->es, 16) + round_up(a.Rsel * 4, 16), 16);
+  // the queued tasks: one block of prune_big_waves() waves per task, 16
+  // waves per CU (at most big_waves blocks: one slot of gcap entries each)
+  pp.lb.waves_per_block = prune_big_waves();
+  pp.lb.blocks = std::min<uint32_t>(idx->num_cu * (16 / pp.lb.waves_per_block), big_waves);
+  pp.lb.lds_bytes = round_up(prune_big_lds_bytes(idx->ad * idx->es, a.Rsel), 16);
   return DDG_OK;
 }
 
@@ -509,7 +519,7 @@ static hipError_t prune_enqueue(ddg_index* idx, const PrunePlan& pp) {
   if (e != hipSuccess) return e;
   a.big_pass = 1;
   a.work = flags + F_PWORK2;
-  a.lds_per_wave = pp.lb.lds_bytes / 4;
+  a.lds_per_wave = pp.lb.lds_bytes;
   a.rrows = 0;
   return launch_prune(idx->dtype, idx->metric, a, pp.lb, idx->stream);
 }

@@ -107,6 +107,9 @@ void search_set_attributes(int max_lds);
 void prune_set_attributes(int max_lds);
 // LDS bytes the Gram-matrix prune path needs per wave
 uint32_t prune_gram_lds_bytes();
+// waves of a k_prune_big block (at most), and its LDS bytes
+constexpr uint32_t kMaxBigWaves = 16;
+uint32_t prune_big_lds_bytes(uint32_t row_bytes, uint32_t Rsel);
 
 // edges.hip
 hipError_t launch_apply_rows(uint32_t* graph, uint32_t R1, const uint32_t* pts, const uint32_t* new_ids,

@@ -15,11 +15,12 @@
 // (graph.h:84-93), else replace G[p] by the pruned list; index.h:309-322).
 //
 // A task whose candidate list exceeds the LDS capacity is queued in
-// big_tasks; a second launch (big_pass = 1) prunes the queued tasks in global
-// scratch: a slot of gcap entries per wave when the host knows the longest
-// list, otherwise a slice from a device bump allocator (big_off) over a pool
-// the host sized so it cannot run out.  Both launches read their task counts
-// on the device, so a round can be recorded before its sizes exist.
+// big_tasks; a second launch (big_pass = 1, k_prune_big) prunes the queued
+// tasks in global scratch, one workgroup per task: a slot of gcap entries per
+// block when the host knows the longest list, otherwise a slice from a device
+// bump allocator (big_off) over a pool the host sized so it cannot run out.
+// Both launches read their task counts on the device, so a round can be
+// recorded before its sizes exist.
 //
 // When a task's candidate rows fit the per-wave LDS row cache (rrows rows),
 // they are staged there once (STAGED): d(p, c) and every d(p*, p') of the
@@ -33,7 +34,7 @@ namespace ddg {
 
 // One robustPrune task (index.h:63-110) with the wave: candidates, sort,
 // greedy selection, write-back.
-template <typename T, int METRIC, bool BIG, bool STAGED>
+template <typename T, int METRIC, bool STAGED>
 __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint32_t p, uint32_t* grow, uint64_t a0,
                                            uint64_t alen, uint32_t gdeg, T* pv, T* sv, uint32_t* sel, float* Cd,
                                            uint32_t* Ci, uint16_t* Cs, T* rows) {
@@ -48,7 +49,6 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
     // ---- candidates: A (given or computed distances) then G[p]
     for (uint32_t j = lane; j < M; j += kWave)
       Ci[j] = j < alen ? a.a_ids[a0 + j] : grow[1 + (j - (uint32_t)alen)];
-    if (BIG) __threadfence_block();
     wave_sync();
     if constexpr (STAGED) {  // candidate j's row to rows[j], eight 16-byte loads in flight per lane
       const uint32_t total = M * nchunks;
@@ -80,7 +80,6 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
       Cd[j] = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(r, pv, ps, a.d, nchunks);
       if (STAGED) Cs[j] = (uint16_t)j;
     }
-    if (BIG) __threadfence_block();
     wave_sync();
 
     // ---- greedy selection (index.h:76-106) without a sort.  Sorting by
@@ -128,7 +127,6 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
         offer(cd, c, o);
       }
       live += popc(km);
-      if (BIG) __threadfence_block();
       wave_sync();
     }
     wave_min();
@@ -164,7 +162,6 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
           keep = !(a.alpha * (double)dsp <= (double)cd);
         }
         const uint64_t km = ballot(keep);
-        if (BIG) __threadfence_block();
         wave_sync();  // every lane has read its entry (and p*'s row slot) before any is overwritten
         if (keep) {
           const uint32_t o = kept + mbcnt(km);
@@ -174,7 +171,6 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
           offer(cd, c, o);
         }
         kept += popc(km);
-        if (BIG) __threadfence_block();
         wave_sync();
       }
       live = kept;
@@ -378,7 +374,7 @@ __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, 
   }
 }
 
-template <typename T, int METRIC, bool BIG>
+template <typename T, int METRIC>
 __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const uint32_t lane = lane_id();
@@ -387,26 +383,17 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
   T* pv = (T*)my;                                   // p's row
   T* sv = (T*)(my + ((a.ad * sizeof(T) + 15u) & ~15u));  // p*'s row
   uint32_t* sel = (uint32_t*)(my + 2u * ((a.ad * sizeof(T) + 15u) & ~15u));
-  float* Cd = (float*)(my + a.off_cand);                       // LDS layout (not BIG)
+  float* Cd = (float*)(my + a.off_cand);
   uint32_t* Ci = (uint32_t*)(Cd + a.ccap);
   uint16_t* Cs = (uint16_t*)(my + a.off_cand + 8u * a.ccap);
   T* rows = (T*)(my + a.off_rows);
   const uint32_t cap = a.ccap;
   uint64_t ntasks = a.np;
-  if (BIG) ntasks = a.nbig[0];
-  else if (a.np_dev && a.np_dev[0] < ntasks) ntasks = a.np_dev[0];
+  if (a.np_dev && a.np_dev[0] < ntasks) ntasks = a.np_dev[0];
 
   for (;;) {
-    const uint32_t tq = fetch_task(a.work);
-    if (tq >= ntasks) break;
-    const uint32_t t = BIG ? a.big_tasks[tq] : tq;
-    if (t == kNoTask) continue;  // fetch_task reconverges the wave first
-    if (BIG) {
-      const uint64_t off = a.big_off ? a.big_off[tq]
-                                     : ((uint64_t)blockIdx.x * (blockDim.x >> 6) + wave) * a.gcap;
-      Cd = a.gscratch_d + off;
-      Ci = a.gscratch_i + off;
-    }
+    const uint32_t t = fetch_task(a.work);
+    if (t >= ntasks) break;
     const uint32_t p = a.pts[t];
     uint32_t* grow = a.graph + (uint64_t)p * a.R1;
     uint64_t a0, alen;
@@ -421,7 +408,7 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
       for (uint32_t j = lane; j < alen; j += kWave) grow[1 + deg + j] = a.a_ids[a0 + j];
       wave_sync();
       if (lane == 0) grow[0] = deg + (uint32_t)alen;
-    } else if (!BIG && M64 > cap) {
+    } else if (M64 > cap) {
       if (lane == 0) {
         const uint32_t s = atomicAdd(a.nbig, 1u);
         if (a.big_off) {
@@ -442,21 +429,183 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
           atomicOr(a.status, 2u);
         }
       }
-    } else if (!BIG && sizeof(T) == 1 && M64 <= a.gram_max) {
+    } else if (sizeof(T) == 1 && M64 <= a.gram_max) {
       prune_task_gram<T, METRIC>(a, t, p, grow, a0, alen, gdeg, pv, sel, Cd, Ci, my + a.off_gram);
-    } else if (!BIG && M64 <= a.rrows) {
-      prune_task<T, METRIC, false, true>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
+    } else if (M64 <= a.rrows) {
+      prune_task<T, METRIC, true>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
     } else {
-      prune_task<T, METRIC, BIG, false>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
+      prune_task<T, METRIC, false>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
     }
     wave_sync();
   }
 }
 
+// The queued (hub) tasks, one per workgroup: all waves of the block share a
+// task's candidate list in global scratch (a slot of gcap entries per block,
+// or the task's slice of the bump pool).  Each greedy step walks the live
+// list in rounds of one candidate per thread: distances to p*, the prune
+// test, then survivors compacted to the front (their order is free: a step
+// takes the live minimum by (distance, id), and entries with equal (distance,
+// id) are copies of one point), and the next minimum by a wave reduction and
+// one over the waves in LDS.  The same arithmetic as prune_task, so the
+// selections are identical; the block only spreads each step's row reads
+// over more lanes.
+template <typename T, int METRIC>
+__global__ __launch_bounds__(1024) void k_prune_big(PruneArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using D = Dist<T, METRIC>;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = bcast0(threadIdx.x >> 6);
+  const uint32_t tid = threadIdx.x, nth = blockDim.x, W = nth >> 6;
+  const uint32_t row_bytes = a.ad * (uint32_t)sizeof(T);
+  const uint32_t nchunks = row_bytes >> 4;
+  const uint32_t rowa = (row_bytes + 15u) & ~15u;
+  T* pv = (T*)smem;                     // p's row
+  T* sv = (T*)(smem + rowa);            // p*'s row
+  uint32_t* sel = (uint32_t*)(smem + 2u * rowa);
+  uint32_t* wcnt = (uint32_t*)(smem + 2u * rowa + ((a.Rsel * 4u + 15u) & ~15u));  // per wave
+  float* wd = (float*)(wcnt + kMaxBigWaves);
+  uint32_t* wi = (uint32_t*)(wd + kMaxBigWaves);
+  uint32_t* wp = wi + kMaxBigWaves;
+  uint32_t* task = wp + kMaxBigWaves;
+  const T* base = (const T*)a.base;
+  const uint64_t ntasks = a.nbig[0];
+
+  float bd = __builtin_inff();
+  uint32_t bi = kPruned, bp = kPruned;  // this lane's smallest (distance, id, position)
+  auto offer = [&](float d, uint32_t i, uint32_t pos) {
+    if (d < bd || (d == bd && (i < bi || (i == bi && pos < bp)))) { bd = d; bi = i; bp = pos; }
+  };
+  // the block's smallest (distance, id, position), on every lane
+  auto block_min = [&]() {
+    for (int o = 1; o < (int)kWave; o <<= 1) {
+      const float od = __shfl_xor(bd, o);
+      const uint32_t oi = __shfl_xor(bi, o), op = __shfl_xor(bp, o);
+      offer(od, oi, op);
+    }
+    if (lane == 0) { wd[wave] = bd; wi[wave] = bi; wp[wave] = bp; }
+    __syncthreads();
+    for (uint32_t w = 0; w < W; w++) offer(wd[w], wi[w], wp[w]);
+    bp = bcast0(bp);
+    bi = bcast0(bi);
+    __syncthreads();  // the slots are read before any later write
+  };
+  // one compaction round: thread tid keeps (c, cd) when keep; survivors go to
+  // Ci / Cd from position `out` on; returns the round's survivors.  Every
+  // entry of the round was read before the barrier, and a survivor's new
+  // position is at most its old one, so no unread entry is overwritten.
+  auto compact = [&](float* Cd, uint32_t* Ci, bool keep, uint32_t c, float cd, uint32_t out) {
+    const uint64_t km = ballot(keep);
+    if (lane == 0) wcnt[wave] = popc(km);
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t w = 0; w < W; w++) {
+      const uint32_t n = wcnt[w];
+      before += w < wave ? n : 0u;
+      total += n;
+    }
+    if (keep) {
+      const uint32_t o = out + before + mbcnt(km);
+      Ci[o] = c;
+      Cd[o] = cd;
+      offer(cd, c, o);
+    }
+    __syncthreads();  // stores and counts done before the next round
+    return total;
+  };
+
+  for (;;) {
+    if (tid == 0) task[0] = atomicAdd(a.work, 1u);
+    __syncthreads();
+    const uint32_t tq = bcast0(task[0]);
+    __syncthreads();
+    if (tq >= ntasks) break;
+    const uint32_t t = a.big_tasks[tq];
+    if (t == kNoTask) continue;
+    const uint64_t off = a.big_off ? a.big_off[tq] : (uint64_t)blockIdx.x * a.gcap;
+    float* Cd = a.gscratch_d + off;
+    uint32_t* Ci = a.gscratch_i + off;
+    const uint32_t p = a.pts[t];
+    uint32_t* grow = a.graph + (uint64_t)p * a.R1;
+    uint64_t a0, alen;
+    if (a.a_off) { a0 = a.a_off[t]; alen = a.a_off[t + 1] - a0; }
+    else { a0 = (uint64_t)t * a.a_stride; alen = a.a_len[t]; }
+    const uint32_t deg = grow[0];
+    const uint32_t gdeg = a.add ? deg : 0u;
+    const uint32_t M = (uint32_t)(alen + gdeg);
+
+    // ---- candidates: A (given or computed distances) then G[p]
+    if (wave == 0) stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
+    for (uint32_t j = tid; j < M; j += nth) Ci[j] = j < alen ? a.a_ids[a0 + j] : grow[1 + (j - (uint32_t)alen)];
+    __syncthreads();
+    const typename D::Q ps = D::prep(pv, nchunks);
+    for (uint32_t j = tid; j < M; j += nth)
+      Cd[j] = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(base + (uint64_t)Ci[j] * a.ad, pv, ps, a.d, nchunks);
+    __syncthreads();
+
+    // ---- greedy selection (index.h:76-106): drop copies of p, then take the
+    // live minimum and prune against it, step by step
+    bd = __builtin_inff();
+    bi = bp = kPruned;
+    uint32_t live = 0;
+    for (uint32_t b0 = 0; b0 < M; b0 += nth) {
+      const uint32_t j = b0 + tid;
+      uint32_t c = 0;
+      float cd = 0.f;
+      if (j < M) { c = Ci[j]; cd = Cd[j]; }
+      live += compact(Cd, Ci, j < M && c != p, c, cd, live);
+    }
+    block_min();
+
+    uint32_t nsel = 0;
+    while (bp != kPruned) {
+      const uint32_t pstar = bi, at = bp;
+      if (tid == 0) sel[nsel] = pstar;
+      nsel++;
+      if (nsel >= a.Rsel) break;  // the reference's remaining prunes cannot change the result
+      if (wave == 0) stage_row(sv, base + (uint64_t)pstar * a.ad, row_bytes);
+      __syncthreads();
+      const typename D::Q ss = D::prep(sv, nchunks);
+      bd = __builtin_inff();
+      bi = bp = kPruned;
+      uint32_t kept = 0;
+      for (uint32_t b0 = 0; b0 < live; b0 += nth) {
+        const uint32_t j = b0 + tid;
+        bool keep = false;
+        uint32_t c = 0;
+        float cd = 0.f;
+        if (j < live && j != at) {
+          c = Ci[j];
+          cd = Cd[j];
+          const float dsp = D::dist(base + (uint64_t)c * a.ad, sv, ss, a.d, nchunks);
+          keep = !(a.alpha * (double)dsp <= (double)cd);
+        }
+        kept += compact(Cd, Ci, keep, c, cd, kept);
+      }
+      live = kept;
+      block_min();  // its barriers also order the reads of sv before the next stage
+    }
+    __syncthreads();
+    if (a.write_graph) {
+      for (uint32_t j = tid; j < nsel; j += nth) grow[1 + j] = sel[j];
+      __syncthreads();
+      if (tid == 0) grow[0] = nsel;
+    } else {
+      for (uint32_t j = tid; j < nsel; j += nth) a.out_ids[(uint64_t)t * a.Rsel + j] = sel[j];
+      if (tid == 0) a.out_len[t] = nsel;
+    }
+    __syncthreads();
+  }
+}
+
+uint32_t prune_big_lds_bytes(uint32_t row_bytes, uint32_t Rsel) {
+  return 2u * ((row_bytes + 15u) & ~15u) + ((Rsel * 4u + 15u) & ~15u) + 4u * kMaxBigWaves * 4u + 16u;
+}
+
 template <typename T, int M>
 static void set_attrs_t(int lds) {
-  (void)hipFuncSetAttribute((const void*)&k_prune<T, M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  (void)hipFuncSetAttribute((const void*)&k_prune<T, M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void*)&k_prune<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void*)&k_prune_big<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
 uint32_t prune_gram_lds_bytes() { return gram_lds_bytes(); }
@@ -475,10 +624,9 @@ void prune_set_attributes(int max_lds) {
 template <typename T, int M>
 static hipError_t launch_t(const PruneArgs& a, const Launch& l, hipStream_t s) {
   if (a.big_pass)
-    hipLaunchKernelGGL((k_prune<T, M, true>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
+    hipLaunchKernelGGL((k_prune_big<T, M>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
   else
-    hipLaunchKernelGGL((k_prune<T, M, false>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s,
-                       a);
+    hipLaunchKernelGGL((k_prune<T, M>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a);
   return hipGetLastError();
 }
 

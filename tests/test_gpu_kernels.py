@@ -86,19 +86,23 @@ def test_reverse_append_or_prune(oracle):
     assert np.array_equal(got, canon(want))
 
 
-def test_prune_lists_beyond_lds(oracle):
-    print("[gpu] prune over candidate lists beyond the LDS capacity", flush=True)
-    Xa, Qa, P, G = _small_case(oracle, "float32", "l2", 12000, 20, seed=9)
-    idx = _index(Xa, 20, G.rows)
+@pytest.mark.parametrize("dtype,metric,d", [("float32", "l2", 20), ("uint8", "l2", 128), ("int8", "mips", 64)])
+def test_prune_lists_beyond_lds(oracle, dtype, metric, d):
+    """Lists of thousands of candidates (with repeated ids) take the queued
+    pass, one workgroup per task."""
+    print(f"[gpu] prune over candidate lists beyond the LDS capacity {dtype} {metric}", flush=True)
+    Xa, Qa, P, G = _small_case(oracle, dtype, metric, 12000, d, seed=9)
+    idx = _index(Xa, d, G.rows, metric)
     rng = np.random.default_rng(3)
     pts = rng.choice(12000, 6, replace=False).astype(np.uint32)
     sizes = np.array([5000, 7000, 100, 9000, 4100, 30])
     off = np.zeros(7, np.uint64)
     off[1:] = np.cumsum(sizes)
     S = [rng.integers(0, 12000, s).astype(np.uint32) for s in sizes]
-    out, ln = _prune_hook(idx, pts, np.concatenate(S), None, None, 0, off, 1.2, 24, 1, 0, 0, int(sizes.max()) + 24)
+    alpha = 1.2 if metric == "l2" else 1.0
+    out, ln = _prune_hook(idx, pts, np.concatenate(S), None, None, 0, off, alpha, 24, 1, 0, 0, int(sizes.max()) + 24)
     for t in range(6):
-        o = oracle.robust_prune(P, G, int(pts[t]), S[t], None, 1.2, True, 24, tie_mode=oracle.TIE_ID)
+        o = oracle.robust_prune(P, G, int(pts[t]), S[t], None, alpha, True, 24, tie_mode=oracle.TIE_ID)
         assert list(out[t, :ln[t]]) == list(o)
 
 
