@@ -207,6 +207,59 @@ __host__ __device__ constexpr uint32_t gram_lds_bytes() { return kGramMax * 8u +
 // walk over bit rows: take the lowest live position, clear its row's bits.
 // Same integers, same float casts and the same double test as the per-pair
 // distances of prune_task, so the selections are identical.
+// One pass over an integer candidate row r: its distance to the staged row q
+// (the bits of Dist<T, METRIC>::dist), its squared norm nn and, for uint8,
+// its byte sum ss; all sums exact in int32.
+template <typename T, int METRIC>
+__device__ __forceinline__ float int_row_terms(const uint4* r, const T* q, const typename Dist<T, METRIC>::Q& qs,
+                                               uint32_t nchunks, int32_t& nn, int32_t& ss) {
+  const uint4* v = (const uint4*)q;
+  if constexpr (sizeof(T) != 1) {  // never reached: the Gram path takes integer rows only
+    nn = ss = 0;
+    return 0.f;
+  } else if constexpr (T(-1) > T(0)) {
+    uint32_t qp = 0, pp = 0, su = 0;
+    for_chunks(r, nchunks, [&](uint4 x, uint32_t c, bool ok) {
+      x = zero_unless(x, ok);
+      const uint4 y = v[c];
+      qp = __builtin_amdgcn_udot4(x.x, y.x, qp, false);
+      qp = __builtin_amdgcn_udot4(x.y, y.y, qp, false);
+      qp = __builtin_amdgcn_udot4(x.z, y.z, qp, false);
+      qp = __builtin_amdgcn_udot4(x.w, y.w, qp, false);
+      pp = __builtin_amdgcn_udot4(x.x, x.x, pp, false);
+      pp = __builtin_amdgcn_udot4(x.y, x.y, pp, false);
+      pp = __builtin_amdgcn_udot4(x.z, x.z, pp, false);
+      pp = __builtin_amdgcn_udot4(x.w, x.w, pp, false);
+      su = __builtin_amdgcn_udot4(x.x, 0x01010101u, su, false);
+      su = __builtin_amdgcn_udot4(x.y, 0x01010101u, su, false);
+      su = __builtin_amdgcn_udot4(x.z, 0x01010101u, su, false);
+      su = __builtin_amdgcn_udot4(x.w, 0x01010101u, su, false);
+    });
+    nn = (int32_t)pp;
+    ss = (int32_t)su;
+    if (METRIC == L2) return (float)(int32_t)(qs.qq + pp - 2u * qp);
+    return -((float)(int32_t)qp);
+  } else {
+    int32_t qp = 0, pp = 0;
+    for_chunks(r, nchunks, [&](uint4 x, uint32_t c, bool ok) {
+      x = zero_unless(x, ok);
+      const uint4 y = v[c];
+      qp = __builtin_amdgcn_sdot4((int)x.x, (int)y.x, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.y, (int)y.y, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.z, (int)y.z, qp, false);
+      qp = __builtin_amdgcn_sdot4((int)x.w, (int)y.w, qp, false);
+      pp = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, pp, false);
+      pp = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, pp, false);
+      pp = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, pp, false);
+      pp = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, pp, false);
+    });
+    nn = pp;
+    ss = 0;
+    if (METRIC == L2) return (float)(qs.qq + pp - 2 * qp);
+    return -((float)qp);
+  }
+}
+
 template <typename T, int METRIC>
 __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, uint32_t p, uint32_t* grow,
                                                 uint64_t a0, uint64_t alen, uint32_t gdeg, T* pv, uint32_t* sel,
@@ -222,23 +275,37 @@ __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, 
   int32_t* Nrm = (int32_t*)gram;
   int32_t* Sm = Nrm + kGramMax;
   uint32_t* Pm = (uint32_t*)(Sm + kGramMax);
+  // until the sort is done, the bit matrix's space holds each candidate's
+  // norm and sum by list position, and the sort carries that position
+  int32_t* NrmL = (int32_t*)Pm;
+  int32_t* SmL = NrmL + kGramMax;
+  uint16_t* Pos = (uint16_t*)(SmL + kGramMax);
   stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
   for (uint32_t j = lane; j < M; j += kWave) Ci[j] = j < alen ? a.a_ids[a0 + j] : grow[1 + (j - (uint32_t)alen)];
   wave_sync();
   const typename D::Q ps = D::prep(pv, nchunks);
-  // distances to p; candidates equal to p sort last and are never taken
+  // one pass over each candidate row: distance to p (unless given), norm and
+  // (uint8) byte sum; candidates equal to p sort last and are never taken
   uint32_t P2 = 1;
   while (P2 < M) P2 <<= 1;
   for (uint32_t j = lane; j < P2; j += kWave) {
     float dj = __builtin_inff();
     uint32_t ij = kPruned;
+    int32_t nn = 0, ss = 0;
     if (j < M) {
       ij = Ci[j];
-      if (ij == p) ij = kPruned;
-      else dj = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(base + (uint64_t)ij * a.ad, pv, ps, a.d, nchunks);
+      if (ij == p) {
+        ij = kPruned;
+      } else {
+        const float dc = int_row_terms<T, METRIC>((const uint4*)(base + (uint64_t)ij * a.ad), pv, ps, nchunks, nn, ss);
+        dj = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : dc;
+      }
     }
     Cd[j] = dj;
     Ci[j] = ij;
+    Pos[j] = (uint16_t)j;
+    NrmL[j] = nn;
+    SmL[j] = ss;
   }
   wave_sync();
   // bitonic sort by (distance, id)
@@ -251,58 +318,50 @@ __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, 
         const float dl = Cd[lo], dh = Cd[hi];
         const uint32_t il = Ci[lo], ih = Ci[hi];
         const bool sw = up ? less_di(dh, ih, dl, il) : less_di(dl, il, dh, ih);
-        if (sw) { Cd[lo] = dh; Cd[hi] = dl; Ci[lo] = ih; Ci[hi] = il; }
+        if (sw) {
+          const uint16_t pl = Pos[lo], ph = Pos[hi];
+          Cd[lo] = dh; Cd[hi] = dl; Ci[lo] = ih; Ci[hi] = il; Pos[lo] = ph; Pos[hi] = pl;
+        }
       }
       wave_sync();
     }
   }
   uint32_t Mp = 0;  // candidates other than p
   for (uint32_t b0 = 0; b0 < M; b0 += kWave) Mp += popc(ballot(b0 + lane < M && Ci[b0 + lane] != kPruned));
-  // norms and sums of the sorted candidates (sums only matter for uint8)
+  // norms and sums in sorted order, then the bit matrix cleared
   for (uint32_t j = lane; j < Mp; j += kWave) {
-    const uint4* r = (const uint4*)(base + (uint64_t)Ci[j] * a.ad);
-    int32_t nn = 0, ss = 0;
-    for (uint32_t c = 0; c < nchunks; c++) {
-      const uint4 x = r[c];
-      if constexpr (U8) {
-        nn = (int32_t)__builtin_amdgcn_udot4(x.x, x.x, (uint32_t)nn, false);
-        nn = (int32_t)__builtin_amdgcn_udot4(x.y, x.y, (uint32_t)nn, false);
-        nn = (int32_t)__builtin_amdgcn_udot4(x.z, x.z, (uint32_t)nn, false);
-        nn = (int32_t)__builtin_amdgcn_udot4(x.w, x.w, (uint32_t)nn, false);
-        ss = (int32_t)__builtin_amdgcn_udot4(x.x, 0x01010101u, (uint32_t)ss, false);
-        ss = (int32_t)__builtin_amdgcn_udot4(x.y, 0x01010101u, (uint32_t)ss, false);
-        ss = (int32_t)__builtin_amdgcn_udot4(x.z, 0x01010101u, (uint32_t)ss, false);
-        ss = (int32_t)__builtin_amdgcn_udot4(x.w, 0x01010101u, (uint32_t)ss, false);
-      } else {
-        nn = __builtin_amdgcn_sdot4((int)x.x, (int)x.x, nn, false);
-        nn = __builtin_amdgcn_sdot4((int)x.y, (int)x.y, nn, false);
-        nn = __builtin_amdgcn_sdot4((int)x.z, (int)x.z, nn, false);
-        nn = __builtin_amdgcn_sdot4((int)x.w, (int)x.w, nn, false);
-      }
-    }
-    Nrm[j] = nn;
-    Sm[j] = ss;
+    const uint32_t o = Pos[j];
+    Nrm[j] = NrmL[o];
+    Sm[j] = SmL[o];
   }
+  wave_sync();
   for (uint32_t i = lane; i < Mp * kGramWords; i += kWave) Pm[i] = 0u;
   wave_sync();
 
   // the upper triangle of the Gram matrix in 16 x 16 tiles; v_mfma_i32_16x16x64_i8
   // leaves row 4 (lane / 16) + r, column lane % 16 in accumulator register r;
   // lane (g4, c16) supplies 16 bytes at element g4 * Q + 16 s of its row for
-  // K step s (the same K order for both operands)
+  // K step s (the same K order for both operands, so a block's rows serve as
+  // either).  The rows of column block J + 1 are loaded while block J is
+  // multiplied; block I + 1's rows then become the next row block.
   const uint32_t g4 = lane >> 4, c16 = lane & 15u;
   const uint32_t Q = a.ad / 4u, nsteps = Q / 16u;
   const uint32_t nb = (Mp + 15u) / 16u;
+  auto load_block = [&](uint32_t J, uint4* v) {
+    const uint32_t rb = 16u * J + c16 < Mp ? 16u * J + c16 : Mp - 1u;
+    const unsigned char* rowb = (const unsigned char*)(base + (uint64_t)Ci[rb] * a.ad);
+#pragma unroll
+    for (uint32_t st = 0; st < 4; st++) v[st] = st < nsteps ? *(const uint4*)(rowb + g4 * Q + 16u * st) : make_uint4(0u, 0u, 0u, 0u);
+  };
+  auto frag = [&](const uint4& v) {
+    return v4i_t{(int)(v.x ^ kFlip), (int)(v.y ^ kFlip), (int)(v.z ^ kFlip), (int)(v.w ^ kFlip)};
+  };
+  uint4 araw[4];
+  if (nb > 0) load_block(0, araw);
   for (uint32_t I = 0; I < nb; I++) {
-    const uint32_t ra = 16u * I + c16 < Mp ? 16u * I + c16 : Mp - 1u;
-    const unsigned char* rowa = (const unsigned char*)(base + (uint64_t)Ci[ra] * a.ad);
     v4i_t af[4];
 #pragma unroll
-    for (uint32_t st = 0; st < 4; st++) {
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (st < nsteps) v = *(const uint4*)(rowa + g4 * Q + 16u * st);
-      af[st] = v4i_t{(int)(v.x ^ kFlip), (int)(v.y ^ kFlip), (int)(v.z ^ kFlip), (int)(v.w ^ kFlip)};
-    }
+    for (uint32_t st = 0; st < 4; st++) af[st] = frag(araw[st]);
     int32_t na[4], sa[4];
 #pragma unroll
     for (uint32_t r = 0; r < 4; r++) {
@@ -310,18 +369,15 @@ __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, 
       na[r] = ar < Mp ? Nrm[ar] : 0;
       sa[r] = ar < Mp ? Sm[ar] : 0;
     }
+    uint4 bcur[4], bnext[4];
+#pragma unroll
+    for (uint32_t st = 0; st < 4; st++) bcur[st] = araw[st];
     for (uint32_t J = I; J < nb; J++) {
-      const uint32_t rb = 16u * J + c16 < Mp ? 16u * J + c16 : Mp - 1u;
-      const unsigned char* rowb = (const unsigned char*)(base + (uint64_t)Ci[rb] * a.ad);
+      if (J + 1u < nb) load_block(J + 1u, bnext);
       v4i_t acc = v4i_t{0, 0, 0, 0};
 #pragma unroll
-      for (uint32_t st = 0; st < 4; st++) {
-        if (st < nsteps) {
-          const uint4 v = *(const uint4*)(rowb + g4 * Q + 16u * st);
-          const v4i_t bv = v4i_t{(int)(v.x ^ kFlip), (int)(v.y ^ kFlip), (int)(v.z ^ kFlip), (int)(v.w ^ kFlip)};
-          acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[st], bv, acc, 0, 0, 0);
-        }
-      }
+      for (uint32_t st = 0; st < 4; st++)
+        if (st < nsteps) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[st], frag(bcur[st]), acc, 0, 0, 0);
       const uint32_t b = 16u * J + c16;
       const int32_t nbv = b < Mp ? Nrm[b] : 0, sbv = b < Mp ? Sm[b] : 0;
       const double cdb = b < Mp ? (double)Cd[b] : 0.0;
@@ -337,6 +393,14 @@ __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, 
           const uint32_t bits = (uint32_t)(m >> (16u * g4)) & 0xFFFFu;
           if (bits) atomicOr(&Pm[ar * kGramWords + (J >> 1)], bits << ((J & 1u) * 16u));
         }
+      }
+      if (J + 1u < nb) {
+        if (J == I) {
+#pragma unroll
+          for (uint32_t st = 0; st < 4; st++) araw[st] = bnext[st];  // the next row block
+        }
+#pragma unroll
+        for (uint32_t st = 0; st < 4; st++) bcur[st] = bnext[st];
       }
     }
   }

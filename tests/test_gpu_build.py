@@ -30,6 +30,22 @@ def test_build_identical_to_oracle(oracle, dtype, metric, d, alpha):
     assert np.array_equal(canon(rows), canon(G.rows))
 
 
+@pytest.mark.parametrize("dtype,metric,d,hi,alpha", [("uint8", "l2", 128, 3, 1.2), ("uint8", "l2", 64, 2, 1.0),
+                                                     ("int8", "mips", 32, 2, 1.0), ("int8", "l2", 256, 128, 1.5)])
+def test_build_integer_ties(oracle, dtype, metric, d, hi, alpha):
+    """Low-entropy integer rows: many equal distances, and prune tests that
+    land exactly on alpha * d(a, b) == d(p, b) (the matrix-core prune decides
+    them by integer thresholds)."""
+    print(f"[gpu] build integer ties {dtype} {metric} d={d} values<{hi} alpha={alpha}", flush=True)
+    from distributed_diskann_amd.formats import align_rows
+    rng = np.random.default_rng(d + hi)
+    lo = 0 if dtype == "uint8" else -hi
+    Xa = align_rows(rng.integers(lo, hi, (3000, d)).astype(dtype))
+    G = oracle.build(oracle.Points(Xa, d, metric), 16, 32, alpha, tie_mode=oracle.TIE_ID)
+    idx, rows, secs = _gpu_build(Xa, d, metric, 16, L=32, alpha=alpha)
+    assert np.array_equal(canon(rows), canon(G.rows))
+
+
 def test_build_n0_identical_and_recall(oracle, n0):
     print("[gpu] build n0 (10k x 20, R32 L50)", flush=True)
     from distributed_diskann_amd.index import recall_with_ties
