@@ -87,7 +87,9 @@ __host__ __device__ __forceinline__ uint32_t filter_bits(uint64_t beam) {
 
 // (distance, id) order of beamSearch.h:59-61.
 __device__ __forceinline__ bool less_di(float da, uint32_t ia, float db, uint32_t ib) {
-  return da < db || (da == db && ia < ib);
+  // both sides evaluated (no short circuit): compares and mask operations,
+  // no branch around the second test
+  return (da < db) | ((da == db) & (ia < ib));
 }
 
 // #entries of the sorted list (d[], id[] & kIdMask) strictly below (x, xi)

@@ -43,7 +43,7 @@ __device__ __forceinline__ bool less_e(uint2 a, uint2 b) {
   return less_di(__uint_as_float(a.x), a.y & kIdMask, __uint_as_float(b.x), b.y & kIdMask);
 }
 __device__ __forceinline__ bool same_e(uint2 a, uint2 b) {
-  return __uint_as_float(a.x) == __uint_as_float(b.x) && (a.y & kIdMask) == (b.y & kIdMask);
+  return (__uint_as_float(a.x) == __uint_as_float(b.x)) & ((a.y & kIdMask) == (b.y & kIdMask));
 }
 
 // Number of entries of the sorted list F[0, n) strictly below c: binary
@@ -54,7 +54,7 @@ __device__ __forceinline__ uint32_t count_below(const uint2* F, uint32_t n, uint
   for (uint32_t step = n ? 1u << (31 - __builtin_clz(n)) : 0u; step; step >>= 1) {
     const uint32_t probe = lo + step;
     const uint2 f = F[(probe <= n ? probe : n) - 1u];
-    if (probe <= n && less_e(f, c)) lo = probe;
+    lo = ((probe <= n) & less_e(f, c)) ? probe : lo;
   }
   return lo;
 }
@@ -65,7 +65,7 @@ __device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t 
   for (uint32_t step = n ? 1u << (31 - __builtin_clz(n)) : 0u; step; step >>= 1) {
     const uint32_t probe = lo + step;
     const uint32_t u = U[(probe <= n ? probe : n) - 1u];
-    if (probe <= n && u <= x) lo = probe;
+    lo = ((probe <= n) & (u <= x)) ? probe : lo;
   }
   return lo;
 }
@@ -378,10 +378,18 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         uint32_t lt = 0, eqb = 0;
         if (v) {
           cj = Cn[j];
-          for (uint32_t t = 0; t < nc; t++) {
-            const uint2 ct = Cn[t];
-            lt += less_e(ct, cj) ? 1u : 0u;
-            eqb += (t < j && same_e(ct, cj)) ? 1u : 0u;
+          // four candidates per step, their LDS reads in flight together
+          // (reads past nc repeat the last candidate and are not counted)
+          for (uint32_t t0 = 0; t0 < nc; t0 += 4) {
+            uint2 ct[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) ct[u] = Cn[t0 + u < nc ? t0 + u : nc - 1u];
+#pragma unroll
+            for (uint32_t u = 0; u < 4; u++) {
+              const uint32_t t = t0 + u;
+              lt += ((t < nc) & less_e(ct[u], cj)) ? 1u : 0u;
+              eqb += ((t < j) & same_e(ct[u], cj)) ? 1u : 0u;
+            }
           }
           Ax[j].x = lt + eqb;
         }
@@ -508,7 +516,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
             if (i < nsz) {
               const uint2 e = F2[i];
               const float ed = __uint_as_float(e.x);
-              over = (thr < ed) || (thr == ed && (e.y & kIdMask) > 0u);
+              over = (thr < ed) | ((thr == ed) & ((e.y & kIdMask) > 0u));
             }
             const uint64_t om = ballot(over);
             if (om) {
