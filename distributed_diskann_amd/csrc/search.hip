@@ -46,6 +46,24 @@ __device__ __forceinline__ bool same_e(uint2 a, uint2 b) {
   return (__uint_as_float(a.x) == __uint_as_float(b.x)) & ((a.y & kIdMask) == (b.y & kIdMask));
 }
 
+// A candidate of one expansion is kept in LDS as a 64-bit key, id in the low
+// word and the distance's bits in the high word, sign-folded so that the
+// unsigned order of the key is the (distance, id) order: one 64-bit compare
+// per pair.  The fold is exact and is undone when a candidate enters the
+// frontier.  It orders -0.0 below +0.0, which the float order calls equal;
+// a search never holds both (L2 distances are sums of squares, +0.0 when
+// zero; inner-product distances are negated sums that start at +0.0, so
+// every zero is -0.0).
+__device__ __forceinline__ uint2 cand_key(float d, uint32_t id) {
+  const uint32_t b = __float_as_uint(d);
+  return make_uint2(id, b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u));
+}
+__device__ __forceinline__ uint2 cand_entry(uint2 k) {  // back to (distance bits, id)
+  const uint32_t o = k.y;
+  return make_uint2(o ^ ((o & 0x80000000u) ? 0x80000000u : 0xFFFFFFFFu), k.x);
+}
+__device__ __forceinline__ uint64_t key64(uint2 k) { return ((uint64_t)k.y << 32) | k.x; }
+
 // Number of entries of the sorted list F[0, n) strictly below c: binary
 // lifting over powers of two, the same number of steps on every lane (n is
 // wave-uniform).
@@ -329,7 +347,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           }
           const bool pass = v && !(dd >= cutoff);
           const uint64_t pm = ballot(pass);
-          if (pass) Cn[nc + mbcnt(pm)] = make_uint2(__float_as_uint(dd), id);
+          if (pass) Cn[nc + mbcnt(pm)] = cand_key(dd, id);
           nc += popc(pm);
           wave_sync();  // the next batch overwrites the staged rows
         }
@@ -345,7 +363,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           const float dd = split_dist<T, METRIC, TRAV>(a, id, v, ql, sub);
           const bool pass = v && sub == 0u && !(dd >= cutoff);
           const uint64_t pm = ballot(pass);
-          if (pass) Cn[nc + mbcnt(pm)] = make_uint2(__float_as_uint(dd), id);
+          if (pass) Cn[nc + mbcnt(pm)] = cand_key(dd, id);
           nc += popc(pm);
         }
       } else {
@@ -360,7 +378,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           }
           const bool pass = v && !(dd >= cutoff);
           const uint64_t pm = ballot(pass);
-          if (pass) Cn[nc + mbcnt(pm)] = make_uint2(__float_as_uint(dd), id);
+          if (pass) Cn[nc + mbcnt(pm)] = cand_key(dd, id);
           nc += popc(pm);
         }
       }
@@ -378,17 +396,20 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         uint32_t lt = 0, eqb = 0;
         if (v) {
           cj = Cn[j];
-          // four candidates per step, their LDS reads in flight together
-          // (reads past nc repeat the last candidate and are not counted)
+          const uint64_t kj = key64(cj);
+          // four candidates per step, their LDS reads in flight together (the
+          // array holds a multiple of 64 entries, so reads past nc stay in
+          // it; those entries are not counted)
           for (uint32_t t0 = 0; t0 < nc; t0 += 4) {
             uint2 ct[4];
 #pragma unroll
-            for (uint32_t u = 0; u < 4; u++) ct[u] = Cn[t0 + u < nc ? t0 + u : nc - 1u];
+            for (uint32_t u = 0; u < 4; u++) ct[u] = Cn[t0 + u];
 #pragma unroll
             for (uint32_t u = 0; u < 4; u++) {
               const uint32_t t = t0 + u;
-              lt += ((t < nc) & less_e(ct[u], cj)) ? 1u : 0u;
-              eqb += ((t < j) & same_e(ct[u], cj)) ? 1u : 0u;
+              const uint64_t kt = key64(ct[u]);
+              lt += ((t < nc) & (kt < kj)) ? 1u : 0u;
+              eqb += ((t < j) & (kt == kj)) ? 1u : 0u;
             }
           }
           Ax[j].x = lt + eqb;
@@ -396,7 +417,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         const uint64_t zm = ballot(v && lt + eqb == 0u);
         if (zm) {
           const int l = first_lane(zm);
-          cmin = make_uint2(readlane_u32(cj.x, l), readlane_u32(cj.y, l));
+          cmin = cand_entry(make_uint2(readlane_u32(cj.x, l), readlane_u32(cj.y, l)));
           have_min = true;
         }
       }
@@ -430,7 +451,8 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         const bool v = j < nc;
         bool emit = false;
         if (v) {
-          const uint2 cj = Cn[j];
+          const uint2 kc = Cn[j];
+          const uint2 cj = cand_entry(kc);
           const uint32_t rank = Ax[j].x;
           const uint32_t lb = count_below(FR, fsz, cj);
           uint32_t eq = 0;
@@ -438,7 +460,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           // equal copies at lower indices: rank - #candidates strictly below
           uint32_t lt = 0;
           if (eq > 0)
-            for (uint32_t t = 0; t < nc; t++) lt += less_e(Cn[t], cj) ? 1u : 0u;
+            for (uint32_t t = 0; t < nc; t++) lt += (key64(Cn[t]) < key64(kc)) ? 1u : 0u;
           emit = eq == 0 || rank - lt >= eq;
           Ax[j] = make_uint2(emit ? rank : (rank | kFlag), lb + eq);
         }
@@ -476,7 +498,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           }
           if (em) {
             Ub[e] = ub;
-            if (e + ub < nsz) NF[e + ub] = Cn[j];
+            if (e + ub < nsz) NF[e + ub] = cand_entry(Cn[j]);
           }
         }
         wave_sync();
