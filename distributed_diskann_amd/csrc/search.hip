@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
     int pos = fsz > 0 ? 0 : -1;
     bool trimmed = false;  // the frontier has been through one union + trim
     // Adjacency row (degree and first 64 neighbors) of the vertex predicted to
-    // be expanded next.  It is requested at the head of an expansion for the
+    // be expanded next.  It is requested after the neighbor filter for the
     // first unvisited frontier entry after the current one, and again after
     // the distances when a new candidate beats that entry; the prediction
     // (the smallest unvisited entry of frontier and new candidates) is the
@@ -238,11 +238,6 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
       const uint2 ue = ui >= 0 ? FR[ui] : make_uint2(0u, 0u);
       pf_id = ui >= 0 ? (ue.y & kIdMask) : 0xFFFFFFFFu;
       pf_table = false;
-      if (pf_id != 0xFFFFFFFFu) {
-        const uint32_t* prow = a.graph + (uint64_t)pf_id * R1;
-        pf_nb = lane < a.R ? prow[1 + lane] : 0u;
-        pf_deg = prow[0];
-      }
 
       // ---- neighbor filter (beamSearch.h:125-131)
       const uint32_t m = deg < a.degree_limit ? deg : a.degree_limit;
@@ -305,6 +300,14 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         wave_sync();
       }
       cmps += nk;
+      // the predicted vertex's adjacency row, requested with the candidate
+      // rows below: the wait for those rows covers it too (requested earlier,
+      // the compiler's waits in the filter would stall on it)
+      if (pf_id != 0xFFFFFFFFu) {
+        const uint32_t* prow = a.graph + (uint64_t)pf_id * R1;
+        pf_nb = lane < a.R ? prow[1 + lane] : 0u;
+        pf_deg = prow[0];
+      }
 
       // ---- distances and cutoff (beamSearch.h:135-145)
       const float cutoff = (fsz < beam) ? 2147483648.0f : __uint_as_float(FR[fsz - 1].x);
@@ -356,7 +359,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         const uint32_t S = a.trav_split;
         const uint32_t sub = lane & (S - 1u);
         const uint32_t step = kWave / S;
-        for (uint32_t b0 = 0; b0 < nk; b0 += step) {
+        auto batch = [&](uint32_t b0) {
           const uint32_t j = b0 + lane / S;
           const bool v = j < nk;
           const uint32_t id = v ? Kid[j] : 0u;
@@ -365,9 +368,13 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           const uint64_t pm = ballot(pass);
           if (pass) Cn[nc + mbcnt(pm)] = cand_key(dd, id);
           nc += popc(pm);
-        }
+        };
+        // the first batch outside the loop: the compiler waits for every
+        // pending load at a loop header, which would stall on the prefetch
+        if (nk > 0) batch(0);
+        for (uint32_t b0 = step; b0 < nk; b0 += step) batch(b0);
       } else {
-        for (uint32_t b0 = 0; b0 < nk; b0 += kWave) {
+        auto batch = [&](uint32_t b0) {
           const uint32_t j = b0 + lane;
           const bool v = j < nk;
           uint32_t id = 0;
@@ -380,7 +387,9 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           const uint64_t pm = ballot(pass);
           if (pass) Cn[nc + mbcnt(pm)] = cand_key(dd, id);
           nc += popc(pm);
-        }
+        };
+        if (nk > 0) batch(0);  // outside the loop, as above
+        for (uint32_t b0 = kWave; b0 < nk; b0 += kWave) batch(b0);
       }
       wave_sync();
 
