@@ -77,6 +77,18 @@ __device__ __forceinline__ uint32_t count_below(const uint2* F, uint32_t n, uint
   return lo;
 }
 
+// count_below for n < 256: eight fixed steps, no loop and no branch.
+__device__ __forceinline__ uint32_t count_below_255(const uint2* F, uint32_t n, uint2 c) {
+  uint32_t lo = 0;
+#pragma unroll
+  for (uint32_t step = 128; step; step >>= 1) {
+    const uint32_t probe = lo + step;
+    const uint2 f = F[(probe <= n ? probe : n) - 1u];
+    lo = ((probe <= n) & less_e(f, c)) ? probe : lo;
+  }
+  return lo;
+}
+
 // Number of entries of the ascending list U[0, n) not above x.
 __device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t n, uint32_t x) {
   uint32_t lo = 0;
@@ -463,9 +475,12 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
           const uint2 kc = Cn[j];
           const uint2 cj = cand_entry(kc);
           const uint32_t rank = Ax[j].x;
-          const uint32_t lb = count_below(FR, fsz, cj);
+          const uint32_t lb = fsz < 256u ? count_below_255(FR, fsz, cj) : count_below(FR, fsz, cj);
           uint32_t eq = 0;
-          while (lb + eq < fsz && same_e(FR[lb + eq], cj)) eq++;
+          if (lb < fsz && same_e(FR[lb], cj)) {  // equal frontier entries (rare)
+            eq = 1;
+            while (lb + eq < fsz && same_e(FR[lb + eq], cj)) eq++;
+          }
           // equal copies at lower indices: rank - #candidates strictly below
           uint32_t lt = 0;
           if (eq > 0)
