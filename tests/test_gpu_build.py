@@ -46,6 +46,25 @@ def test_build_integer_ties(oracle, dtype, metric, d, hi, alpha):
     assert np.array_equal(canon(rows), canon(G.rows))
 
 
+def test_build_hub_identical(oracle):
+    """A star: points on a shell around a centre point, which is among every
+    point's nearest neighbours, so its reverse-edge lists in one batch run to
+    thousands of candidates and take the queued one-workgroup-per-task pass,
+    writing the graph from bump-allocated scratch."""
+    print("[gpu] build star (hub reverse lists) 20000 x 16", flush=True)
+    from distributed_diskann_amd.formats import align_rows
+    rng = np.random.default_rng(21)
+    n, d = 20000, 16
+    dirs = rng.standard_normal((n, d))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    X = (dirs * (1.0 + 0.05 * rng.standard_normal((n, 1)))).astype(np.float32)
+    X[0] = 0.0
+    Xa = align_rows(X)
+    G = oracle.build(oracle.Points(Xa, d), 16, 32, 1.2, tie_mode=oracle.TIE_ID)
+    idx, rows, secs = _gpu_build(Xa, d, "l2", 16, L=32, alpha=1.2)
+    assert np.array_equal(canon(rows), canon(G.rows))
+
+
 def test_build_n0_identical_and_recall(oracle, n0):
     print("[gpu] build n0 (10k x 20, R32 L50)", flush=True)
     from distributed_diskann_amd.index import recall_with_ties
