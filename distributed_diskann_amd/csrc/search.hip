@@ -138,14 +138,14 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
 
     const uint32_t beam = a.per_query_beam ? a.per_query_beam[q] : a.beam;
     const uint32_t mask = (1u << filter_bits(beam)) - 1u;
-    int64_t self_id;
+    uint32_t self_id;  // ids and query indices are below 2^32
     const T* qsrc;
     if (a.build_mode) {
       uint32_t pid = a.qids[q];
-      self_id = (int64_t)pid;
+      self_id = pid;
       qsrc = base + (uint64_t)pid * a.ad;
     } else {
-      self_id = a.qids ? (int64_t)a.qids[q] : (int64_t)q;
+      self_id = a.qids ? a.qids[q] : q;
       qsrc = (const T*)a.queries + (uint64_t)q * a.ad;
     }
     stage_traversal_query<T, METRIC, TRAV>(a, ql, qsrc);
@@ -190,7 +190,8 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
     uint32_t fsz = ns;
     uint32_t cur = 0;
     uint64_t cmps = ns;
-    uint64_t nvis = 0;
+    uint32_t nvis = 0;
+    const uint32_t limit = a.limit < 0xFFFFFFFFull ? (uint32_t)a.limit : 0xFFFFFFFFu;
     int pos = fsz > 0 ? 0 : -1;
     bool trimmed = false;  // the frontier has been through one union + trim
     // Adjacency row (degree and first 64 neighbors) of the vertex predicted to
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
     uint32_t pf_id = 0xFFFFFFFFu, pf_deg = 0, pf_nb = 0, pf_prev = 0xFFFFFFFFu, pf_slot = 0;
     bool pf_table = false;
 
-    while (pos >= 0 && nvis < a.limit) {
+    while (pos >= 0 && nvis < limit) {
       uint2* FR = cur ? F1 : F0;
       const uint2 ce = FR[pos];
       const float cdist = __uint_as_float(ce.x);
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         const uint32_t j = b0 + lane;
         const bool valid = j < m;
         const uint32_t av = valid ? (b0 == 0 ? nb0 : row[1 + j]) : 0u;
-        const bool act = valid && ((int64_t)av != self_id);
+        const bool act = valid && (av != self_id);
         const bool pre = b0 == 0 && use_pf_table;  // slot and table word loaded ahead
         const uint32_t slot = pre ? pf_slot : (uint32_t)hash64_2((uint64_t)av) & mask;
         // table stores of the previous 64 neighbors by other lanes (those of
