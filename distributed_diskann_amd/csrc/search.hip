@@ -239,13 +239,23 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
       // the first unvisited frontier entry after pos (every entry before pos
       // is visited): the next vertex to expand unless a new candidate beats it
       int ui = -1;
-      for (uint32_t b0 = (uint32_t)pos + 1u; b0 < fsz; b0 += kWave) {
+      auto unvisited = [&](uint32_t b0) {
         const uint32_t i = b0 + lane;
-        const bool un = i < fsz && !(FR[i].y & kFlag);
-        const uint64_t um = ballot(un);
-        if (um) {
-          ui = (int)(b0 + (uint32_t)first_lane(um));
-          break;
+        return ballot(i < fsz && !(FR[i].y & kFlag));
+      };
+      {  // the first 64 entries outside the loop (no loop header to wait at)
+        const uint32_t b1 = (uint32_t)pos + 1u;
+        const uint64_t um0 = unvisited(b1);
+        if (um0) {
+          ui = (int)(b1 + (uint32_t)first_lane(um0));
+        } else {
+          for (uint32_t b0 = b1 + kWave; b0 < fsz; b0 += kWave) {
+            const uint64_t um = unvisited(b0);
+            if (um) {
+              ui = (int)(b0 + (uint32_t)first_lane(um));
+              break;
+            }
+          }
         }
       }
       const uint2 ue = ui >= 0 ? FR[ui] : make_uint2(0u, 0u);
@@ -556,8 +566,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
         if (nsz > beam) nsz = beam;
         if (a.k > 0 && nsz > a.k && a.cut_trim) {
           const float thr = (float)(a.cut * (double)__uint_as_float(F2[a.k].x));
-          uint32_t cut_at = nsz;
-          for (uint32_t b0 = 0; b0 < nsz; b0 += kWave) {
+          auto over_mask = [&](uint32_t b0) {
             const uint32_t i = b0 + lane;
             bool over = false;
             if (i < nsz) {
@@ -565,10 +574,20 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(S
               const float ed = __uint_as_float(e.x);
               over = (thr < ed) | ((thr == ed) & ((e.y & kIdMask) > 0u));
             }
-            const uint64_t om = ballot(over);
-            if (om) {
-              cut_at = b0 + (uint32_t)first_lane(om);
-              break;
+            return ballot(over);
+          };
+          // the first 64 entries outside the loop (no loop header to wait at)
+          uint32_t cut_at = nsz;
+          const uint64_t om0 = over_mask(0);
+          if (om0) {
+            cut_at = (uint32_t)first_lane(om0);
+          } else {
+            for (uint32_t b0 = kWave; b0 < nsz; b0 += kWave) {
+              const uint64_t om = over_mask(b0);
+              if (om) {
+                cut_at = b0 + (uint32_t)first_lane(om);
+                break;
+              }
             }
           }
           nsz = cut_at;
