@@ -144,6 +144,41 @@ def test_search_vs_oracle_dtypes(oracle, dtype, metric, d):
         assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
 
 
+@pytest.mark.parametrize("dtype,metric", [("float32", "mips"), ("int8", "mips"), ("float32", "l2"), ("uint8", "l2")])
+def test_search_zero_distances(oracle, dtype, metric):
+    """Many exact zero distances: inner products of vectors with disjoint
+    supports (every zero is -0.0) and, for L2, base rows that repeat the
+    queries (+0.0).  Candidates are ranked by integer keys of the distance
+    bits, so zeros and their ties must come out exactly as the float order."""
+    print(f"[gpu] search zero distances {dtype} {metric}", flush=True)
+    from distributed_diskann_amd.formats import align_rows
+    rng = np.random.default_rng(17)
+    n, d, nq = 3000, 32, 200
+    if dtype == "float32":
+        X = rng.standard_normal((n, d)).astype(np.float32)
+        Q = rng.standard_normal((nq, d)).astype(np.float32)
+    else:
+        lo, hi = (0, 4) if dtype == "uint8" else (-3, 4)
+        X = rng.integers(lo, hi, (n, d)).astype(dtype)
+        Q = rng.integers(lo, hi, (nq, d)).astype(dtype)
+    if metric == "mips":
+        X[: n // 2, : d // 2] = 0  # half the base is orthogonal to every query
+        Q[:, d // 2:] = 0
+    else:
+        X[:nq] = Q  # exact copies of the queries
+        X[nq:2 * nq] = Q
+    Xa, Qa = align_rows(X), align_rows(Q)
+    P = oracle.Points(Xa, d, metric)
+    G = oracle.build(P, 16, 32, 1.2 if metric == "l2" else 1.0, tie_mode=oracle.TIE_ID)
+    idx = _index(Xa, d, G.rows, metric)
+    for Ls in (10, 50):
+        ids, dists, cmps, vis = idx.search(Qa, 10, Ls, 1.35)
+        oi, od, oc, ov = oracle.search_all(P, G, Qa, 10, Ls, cut=1.35)
+        assert np.array_equal(ids, oi)
+        assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+        assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+
+
 def test_search_limits_and_starts(oracle, xs):
     print("[gpu] limits / degree limits / starts", flush=True)
     idx = _index(xs["X"], xs["d"], xs["G"])
