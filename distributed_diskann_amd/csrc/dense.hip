@@ -119,7 +119,8 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kGtRows = 16;    // queries per wave
-constexpr uint32_t kGtTile = 64;    // points per tile
+constexpr uint32_t kGtTile = 64;    // points per tile (int8 path)
+constexpr uint32_t kGtTileF = 32;   // points per tile (FP64 path: two blocks fit a CU)
 constexpr uint32_t kGtMaxQf = 64;   // FP64 path: dims per lane group (d <= 256)
 constexpr uint32_t kGtMaxSteps = 8; // int8 path: K steps of 64 (d <= 512)
 
@@ -146,8 +147,9 @@ struct GtTopk {
   uint32_t* ci;
   uint32_t* thr_i;
   uint32_t* cnt;
-  uint32_t cap, k;
-  __device__ GtTopk(unsigned char* base, uint32_t cap_, uint32_t k_) : cap(cap_), k(k_) {
+  uint32_t cap, k, tile;  // tile: points offered between two maintain() calls
+  __device__ GtTopk(unsigned char* base, uint32_t cap_, uint32_t k_, uint32_t tile_)
+      : cap(cap_), k(k_), tile(tile_) {
     cd = (double*)base;
     thr_d = cd + kGtRows * cap;
     ci = (uint32_t*)(thr_d + kGtRows);
@@ -202,7 +204,7 @@ struct GtTopk {
   __device__ void maintain() {
     wave_sync();
     const uint32_t l = lane_id();
-    uint64_t need = ballot(l < kGtRows && cnt[l] + kGtTile > cap);
+    uint64_t need = ballot(l < kGtRows && cnt[l] + tile > cap);
     while (need) {
       const int r = first_lane(need);
       need &= need - 1;
@@ -226,26 +228,28 @@ struct GtTopk {
 };
 
 template <int METRIC>
-__global__ __launch_bounds__(256) void k_gt_f64(GtArgs g) {
+__global__ __launch_bounds__(256, 2) void k_gt_f64(GtArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* tile_pn = (double*)smem;
   float* Bs = (float*)(smem + gt_tile_off(g.cap));
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t g4 = lane >> 4, c16 = lane & 15u;
-  GtTopk tk(smem + 1024u + wave * gt_wave_bytes(g.cap), g.cap, g.k);
+  GtTopk tk(smem + 1024u + wave * gt_wave_bytes(g.cap), g.cap, g.k, kGtTileF);
   tk.init();
   const uint64_t q0 = (uint64_t)blockIdx.x * (4u * kGtRows) + wave * kGtRows;
   const uint32_t Q = g.Q;
 
-  // A operand: query q0 + c16, dimensions g4 Q + s, held in registers
-  double af[kGtMaxQf];
+  // A operand: query q0 + c16, dimensions g4 Q + s, held in registers as
+  // float (widened to double exactly at each use: half the registers, so two
+  // blocks fit a CU)
+  float af[kGtMaxQf];
   {
     const uint64_t qa = q0 + c16 < g.nq ? q0 + c16 : g.nq - 1;
     const float* qr = (const float*)g.queries + qa * g.ad;
 #pragma unroll
     for (uint32_t s = 0; s < kGtMaxQf; s++) {
       const uint32_t j = g4 * Q + s;
-      af[s] = (s < Q && j < g.d) ? (double)qr[j] : 0.0;
+      af[s] = (s < Q && j < g.d) ? qr[j] : 0.f;
     }
   }
   // v_mfma_f64_16x16x4_f64 leaves row 4 r + lane / 16, column lane % 16 in
@@ -260,11 +264,11 @@ __global__ __launch_bounds__(256) void k_gt_f64(GtArgs g) {
   const uint64_t p_begin = (uint64_t)blockIdx.y * g.per_slice;
   const uint64_t p_end = p_begin + g.per_slice < g.n ? p_begin + g.per_slice : g.n;
   const float* X = (const float*)g.base;
-  for (uint64_t t0 = p_begin; t0 < p_end; t0 += kGtTile) {
+  for (uint64_t t0 = p_begin; t0 < p_end; t0 += kGtTileF) {
     __syncthreads();
     // 16-byte chunks of the tile rows (the device rows are zero beyond d),
     // eight loads in flight per thread before the LDS stores
-    const uint32_t total = kGtTile * Q;
+    const uint32_t total = kGtTileF * Q;
     for (uint32_t e0 = threadIdx.x; e0 < total; e0 += 8u * blockDim.x) {
       float4 v[8];
 #pragma unroll
@@ -290,25 +294,25 @@ __global__ __launch_bounds__(256) void k_gt_f64(GtArgs g) {
         }
       }
     }
-    if (threadIdx.x < kGtTile) {
+    if (threadIdx.x < kGtTileF) {
       const uint64_t p = t0 + threadIdx.x;
       tile_pn[threadIdx.x] = (METRIC == L2 && p < p_end) ? g.pn[p] : 0.0;
     }
     __syncthreads();
-    v4d acc[4];
+    v4d acc[kGtTileF / 16];
 #pragma unroll
-    for (uint32_t b = 0; b < 4; b++) acc[b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = 0; b < kGtTileF / 16; b++) acc[b] = v4d{0.0, 0.0, 0.0, 0.0};
     const float* bp = Bs + c16 * g.bstride + g4 * Q;
 #pragma unroll
     for (uint32_t s = 0; s < kGtMaxQf; s++) {
       if (s < Q) {
 #pragma unroll
-        for (uint32_t b = 0; b < 4; b++)
-          acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[s], (double)bp[16u * b * g.bstride + s], acc[b], 0, 0, 0);
+        for (uint32_t b = 0; b < kGtTileF / 16; b++)
+          acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)af[s], (double)bp[16u * b * g.bstride + s], acc[b], 0, 0, 0);
       }
     }
 #pragma unroll
-    for (uint32_t b = 0; b < 4; b++) {
+    for (uint32_t b = 0; b < kGtTileF / 16; b++) {
       const uint32_t pl = 16u * b + c16;
       const uint64_t p = t0 + pl;
 #pragma unroll
@@ -334,7 +338,7 @@ __global__ __launch_bounds__(256) void k_gt_i8(GtArgs g) {
   unsigned char* Bs = smem + gt_tile_off(g.cap);
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t g4 = lane >> 4, c16 = lane & 15u;
-  GtTopk tk(smem + 1024u + wave * gt_wave_bytes(g.cap), g.cap, g.k);
+  GtTopk tk(smem + 1024u + wave * gt_wave_bytes(g.cap), g.cap, g.k, kGtTile);
   tk.init();
   const uint64_t q0 = (uint64_t)blockIdx.x * (4u * kGtRows) + wave * kGtRows;
   const uint32_t Q = g.Q, Q4 = 4u * Q, nsteps = Q / 16u, cpr = Q4 / 16u;
@@ -486,13 +490,14 @@ struct GtPlan {
 
 static GtPlan gt_plan(int dtype, uint64_t n, uint64_t nq, uint32_t d, uint32_t k) {
   GtPlan p{};
-  p.cap = kGtTile + (k > 16 ? 32u : 16u);
+  const uint32_t tile = dtype == F32 ? kGtTileF : kGtTile;
+  p.cap = tile + (k > 16 ? 32u : 16u);
   const uint32_t qd = (d + 3) / 4;
   if (dtype == F32) {
     p.mfma = qd <= kGtMaxQf;
     p.Q = qd;
     p.bstride = 4 * qd + 1;
-    p.lds = gt_tile_off(p.cap) + kGtTile * p.bstride * 4;
+    p.lds = gt_tile_off(p.cap) + kGtTileF * p.bstride * 4;
   } else {
     p.Q = (qd + 15) / 16 * 16;
     p.mfma = p.Q / 16 <= kGtMaxSteps;
@@ -500,13 +505,13 @@ static GtPlan gt_plan(int dtype, uint64_t n, uint64_t nq, uint32_t d, uint32_t k
     p.lds = gt_tile_off(p.cap) + kGtTile * p.bstride;
   }
   const uint64_t qtiles = (nq + 4 * kGtRows - 1) / (4 * kGtRows);
-  const uint64_t tiles = (n + kGtTile - 1) / kGtTile;
+  const uint64_t tiles = (n + tile - 1) / tile;
   uint64_t S = (2048 + qtiles - 1) / qtiles;
   S = std::min<uint64_t>(S, std::max<uint64_t>(1, 1024 / k));
   S = std::max<uint64_t>(1, std::min<uint64_t>(S, tiles));
   const uint64_t per_tiles = (tiles + S - 1) / S;
   p.S = (uint32_t)((tiles + per_tiles - 1) / per_tiles);
-  p.per_slice = per_tiles * kGtTile;
+  p.per_slice = per_tiles * tile;
   return p;
 }
 
