@@ -119,8 +119,9 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 
 constexpr uint32_t kGtRows = 16;    // queries per wave
-constexpr uint32_t kGtTile = 64;    // points per tile (int8 path)
-constexpr uint32_t kGtTileF = 32;   // points per tile (FP64 path: two blocks fit a CU)
+constexpr uint32_t kGtTile = 32;    // points per tile
+constexpr uint32_t kGtWavesF = 4;   // waves per block, FP64 path
+constexpr uint32_t kGtWavesI8 = 8;  // waves per block, int8 path: 128 queries share each tile
 constexpr uint32_t kGtMaxQf = 64;   // FP64 path: dims per lane group (d <= 256)
 constexpr uint32_t kGtMaxSteps = 8; // int8 path: K steps of 64 (d <= 512)
 
@@ -139,7 +140,7 @@ struct GtArgs {
 
 // per-wave candidate lists: cd[16][cap] f64 | thr_d[16] f64 | ci[16][cap] u32 | thr_i[16] u32 | cnt[16] u32
 __host__ __device__ inline uint32_t gt_wave_bytes(uint32_t cap) { return 192u * cap + 256u; }
-__host__ __device__ inline uint32_t gt_tile_off(uint32_t cap) { return 1024u + 4u * gt_wave_bytes(cap); }
+__host__ __device__ inline uint32_t gt_tile_off(uint32_t cap, uint32_t waves) { return 1024u + waves * gt_wave_bytes(cap); }
 
 struct GtTopk {
   double* cd;
@@ -231,12 +232,12 @@ template <int METRIC>
 __global__ __launch_bounds__(256, 2) void k_gt_f64(GtArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* tile_pn = (double*)smem;
-  float* Bs = (float*)(smem + gt_tile_off(g.cap));
+  float* Bs = (float*)(smem + gt_tile_off(g.cap, kGtWavesF));
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t g4 = lane >> 4, c16 = lane & 15u;
-  GtTopk tk(smem + 1024u + wave * gt_wave_bytes(g.cap), g.cap, g.k, kGtTileF);
+  GtTopk tk(smem + 1024u + wave * gt_wave_bytes(g.cap), g.cap, g.k, kGtTile);
   tk.init();
-  const uint64_t q0 = (uint64_t)blockIdx.x * (4u * kGtRows) + wave * kGtRows;
+  const uint64_t q0 = (uint64_t)blockIdx.x * (kGtWavesF * kGtRows) + wave * kGtRows;
   const uint32_t Q = g.Q;
 
   // A operand: query q0 + c16, dimensions g4 Q + s, held in registers as
@@ -264,11 +265,11 @@ __global__ __launch_bounds__(256, 2) void k_gt_f64(GtArgs g) {
   const uint64_t p_begin = (uint64_t)blockIdx.y * g.per_slice;
   const uint64_t p_end = p_begin + g.per_slice < g.n ? p_begin + g.per_slice : g.n;
   const float* X = (const float*)g.base;
-  for (uint64_t t0 = p_begin; t0 < p_end; t0 += kGtTileF) {
+  for (uint64_t t0 = p_begin; t0 < p_end; t0 += kGtTile) {
     __syncthreads();
     // 16-byte chunks of the tile rows (the device rows are zero beyond d),
     // eight loads in flight per thread before the LDS stores
-    const uint32_t total = kGtTileF * Q;
+    const uint32_t total = kGtTile * Q;
     for (uint32_t e0 = threadIdx.x; e0 < total; e0 += 8u * blockDim.x) {
       float4 v[8];
 #pragma unroll
@@ -294,25 +295,25 @@ __global__ __launch_bounds__(256, 2) void k_gt_f64(GtArgs g) {
         }
       }
     }
-    if (threadIdx.x < kGtTileF) {
+    if (threadIdx.x < kGtTile) {
       const uint64_t p = t0 + threadIdx.x;
       tile_pn[threadIdx.x] = (METRIC == L2 && p < p_end) ? g.pn[p] : 0.0;
     }
     __syncthreads();
-    v4d acc[kGtTileF / 16];
+    v4d acc[kGtTile / 16];
 #pragma unroll
-    for (uint32_t b = 0; b < kGtTileF / 16; b++) acc[b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (uint32_t b = 0; b < kGtTile / 16; b++) acc[b] = v4d{0.0, 0.0, 0.0, 0.0};
     const float* bp = Bs + c16 * g.bstride + g4 * Q;
 #pragma unroll
     for (uint32_t s = 0; s < kGtMaxQf; s++) {
       if (s < Q) {
 #pragma unroll
-        for (uint32_t b = 0; b < kGtTileF / 16; b++)
+        for (uint32_t b = 0; b < kGtTile / 16; b++)
           acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)af[s], (double)bp[16u * b * g.bstride + s], acc[b], 0, 0, 0);
       }
     }
 #pragma unroll
-    for (uint32_t b = 0; b < kGtTileF / 16; b++) {
+    for (uint32_t b = 0; b < kGtTile / 16; b++) {
       const uint32_t pl = 16u * b + c16;
       const uint64_t p = t0 + pl;
 #pragma unroll
@@ -329,18 +330,18 @@ __global__ __launch_bounds__(256, 2) void k_gt_f64(GtArgs g) {
 }
 
 template <typename T, int METRIC>
-__global__ __launch_bounds__(256) void k_gt_i8(GtArgs g) {
+__global__ __launch_bounds__(64 * kGtWavesI8) void k_gt_i8(GtArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool U8 = sizeof(T) == 1 && T(-1) > T(0);
   constexpr uint32_t kFlip = U8 ? 0x80808080u : 0u;
   double* tile_pn = (double*)smem;
   double* tile_ps = tile_pn + kGtTile;
-  unsigned char* Bs = smem + gt_tile_off(g.cap);
+  unsigned char* Bs = smem + gt_tile_off(g.cap, kGtWavesI8);
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t g4 = lane >> 4, c16 = lane & 15u;
   GtTopk tk(smem + 1024u + wave * gt_wave_bytes(g.cap), g.cap, g.k, kGtTile);
   tk.init();
-  const uint64_t q0 = (uint64_t)blockIdx.x * (4u * kGtRows) + wave * kGtRows;
+  const uint64_t q0 = (uint64_t)blockIdx.x * (kGtWavesI8 * kGtRows) + wave * kGtRows;
   const uint32_t Q = g.Q, Q4 = 4u * Q, nsteps = Q / 16u, cpr = Q4 / 16u;
   const double P = (double)Q4;
 
@@ -400,22 +401,22 @@ __global__ __launch_bounds__(256) void k_gt_i8(GtArgs g) {
       tile_ps[threadIdx.x] = p < p_end ? g.psum[p] : 0.0;
     }
     __syncthreads();
-    v4i acc[4];
+    v4i acc[kGtTile / 16];
 #pragma unroll
-    for (uint32_t b = 0; b < 4; b++) acc[b] = v4i{0, 0, 0, 0};
+    for (uint32_t b = 0; b < kGtTile / 16; b++) acc[b] = v4i{0, 0, 0, 0};
     const unsigned char* bp = Bs + c16 * g.bstride + g4 * Q;
 #pragma unroll
     for (uint32_t s = 0; s < kGtMaxSteps; s++) {
       if (s < nsteps) {
 #pragma unroll
-        for (uint32_t b = 0; b < 4; b++) {
+        for (uint32_t b = 0; b < kGtTile / 16; b++) {
           const v4i bv = *(const v4i*)(bp + 16u * b * g.bstride + 16u * s);
           acc[b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[s], bv, acc[b], 0, 0, 0);
         }
       }
     }
 #pragma unroll
-    for (uint32_t b = 0; b < 4; b++) {
+    for (uint32_t b = 0; b < kGtTile / 16; b++) {
       const uint32_t pl = 16u * b + c16;
       const uint64_t p = t0 + pl;
 #pragma unroll
@@ -490,28 +491,29 @@ struct GtPlan {
 
 static GtPlan gt_plan(int dtype, uint64_t n, uint64_t nq, uint32_t d, uint32_t k) {
   GtPlan p{};
-  const uint32_t tile = dtype == F32 ? kGtTileF : kGtTile;
-  p.cap = tile + (k > 16 ? 32u : 16u);
+  p.cap = kGtTile + (k > 16 ? 32u : 16u);
   const uint32_t qd = (d + 3) / 4;
   if (dtype == F32) {
     p.mfma = qd <= kGtMaxQf;
     p.Q = qd;
     p.bstride = 4 * qd + 1;
-    p.lds = gt_tile_off(p.cap) + kGtTileF * p.bstride * 4;
+    p.lds = gt_tile_off(p.cap, kGtWavesF) + kGtTile * p.bstride * 4;
   } else {
     p.Q = (qd + 15) / 16 * 16;
     p.mfma = p.Q / 16 <= kGtMaxSteps;
     p.bstride = 4 * p.Q + 16;
-    p.lds = gt_tile_off(p.cap) + kGtTile * p.bstride;
+    p.lds = gt_tile_off(p.cap, kGtWavesI8) + kGtTile * p.bstride;
   }
-  const uint64_t qtiles = (nq + 4 * kGtRows - 1) / (4 * kGtRows);
-  const uint64_t tiles = (n + tile - 1) / tile;
+  if (p.lds > 160u * 1024u) p.mfma = false;  // beyond a CU's LDS: the per-query kernel
+  const uint32_t qb = (dtype == F32 ? kGtWavesF : kGtWavesI8) * kGtRows;  // queries per block
+  const uint64_t qtiles = (nq + qb - 1) / qb;
+  const uint64_t tiles = (n + kGtTile - 1) / kGtTile;
   uint64_t S = (2048 + qtiles - 1) / qtiles;
   S = std::min<uint64_t>(S, std::max<uint64_t>(1, 1024 / k));
   S = std::max<uint64_t>(1, std::min<uint64_t>(S, tiles));
   const uint64_t per_tiles = (tiles + S - 1) / S;
   p.S = (uint32_t)((tiles + per_tiles - 1) / per_tiles);
-  p.per_slice = per_tiles * tile;
+  p.per_slice = per_tiles * kGtTile;
   return p;
 }
 
@@ -562,13 +564,14 @@ static hipError_t gt_t(const void* base, uint64_t n, const void* queries, uint64
                      (double*)g.pn, (double*)g.psum);
   hipLaunchKernelGGL((k_gt_norms<T>), dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s, (const T*)queries, nq, d,
                      ad, (double*)g.qn, (double*)g.qsum);
-  const dim3 grid((uint32_t)((nq + 4 * kGtRows - 1) / (4 * kGtRows)), p.S);
+  const uint32_t waves = sizeof(T) == 4 ? kGtWavesF : kGtWavesI8;
+  const dim3 grid((uint32_t)((nq + waves * kGtRows - 1) / (waves * kGtRows)), p.S);
   if constexpr (sizeof(T) == 4) {
     (void)hipFuncSetAttribute((const void*)&k_gt_f64<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
-    hipLaunchKernelGGL((k_gt_f64<M>), grid, dim3(256), p.lds, s, g);
+    hipLaunchKernelGGL((k_gt_f64<M>), grid, dim3(64 * kGtWavesF), p.lds, s, g);
   } else {
     (void)hipFuncSetAttribute((const void*)&k_gt_i8<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
-    hipLaunchKernelGGL((k_gt_i8<T, M>), grid, dim3(256), p.lds, s, g);
+    hipLaunchKernelGGL((k_gt_i8<T, M>), grid, dim3(64 * kGtWavesI8), p.lds, s, g);
   }
   const uint32_t E = p.S * k;
   const uint32_t wpb = 4, mlds = wpb * ((E * 12u + 15u) & ~15u);
