@@ -104,9 +104,11 @@ __device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t 
 // launch choice for long rows; a separate instantiation, so the other
 // kernels do not carry its registers).  The native kernels are compiled for
 // at least 6 waves per SIMD (80 VGPRs): measured faster than the compiler's
-// own choice, which hoists every staged query chunk of a distance.
+// own choice, which hoists every staged query chunk of a distance.  The BF16
+// kernels take 5 (96 VGPRs; 1.39 -> 1.23 ms on 100K x 20); FP8 keeps the
+// compiler's choice (no gain at 5).
 template <typename T, int METRIC, int TRAV, bool GTAB, bool STAGE>
-__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : 1)) void k_search(SearchArgs a) {
+__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 ? 5 : 1))) void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using P = TravDist<T, METRIC, TRAV>;
   const uint32_t lane = lane_id();
