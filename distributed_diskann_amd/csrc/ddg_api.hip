@@ -194,14 +194,29 @@ static uint32_t tune_key(int trav, uint32_t bits) { return ((uint32_t)trav << 8)
 // shape being timed, a tuned shape for (trav, filter bits), the defaults
 // (filter in LDS up to 8 KB, up to 4 waves per block, as many blocks per CU
 // as LDS and 32 waves allow).  Returns false when the shape does not fit.
+static bool search_layout_cfg(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, int trav,
+                              const ddg_index::Tuned* cfg, SearchArgs& a, Launch& l);
+
 static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, int trav, SearchArgs& a,
                           Launch& l, bool use_tuned = true) {
-  const uint32_t qrow = trav_layout(idx, trav, a);
+  if (idx->cfg_override) return search_layout_cfg(idx, beam_max, nstarts, trav, idx->cfg_override, a, l);
   const uint32_t bits = filter_bits(beam_max);
-  const ddg_index::Tuned* cfg = idx->cfg_override;
-  if (!cfg && use_tuned)
+  const ddg_index::Tuned* cfg = nullptr;
+  if (use_tuned)
     for (const auto& t : idx->tuned)
       if (t.key == tune_key(trav, bits)) cfg = &t;
+  // a shape tuned at another beam or start count of the same filter class may
+  // not fit this frontier; the default shape is then used
+  if (cfg && search_layout_cfg(idx, beam_max, nstarts, trav, cfg, a, l)) return true;
+  return search_layout_cfg(idx, beam_max, nstarts, trav, nullptr, a, l);
+}
+
+static bool search_layout_cfg(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, int trav,
+                              const ddg_index::Tuned* cfg, SearchArgs& a, Launch& l) {
+  a = SearchArgs{};
+  l = Launch{};
+  const uint32_t qrow = trav_layout(idx, trav, a);
+  const uint32_t bits = filter_bits(beam_max);
   const bool gtab = cfg ? cfg->gtab != 0 : global_table(bits);
   const uint32_t table = gtab ? 0u : 4u << bits;
   a.gtable = nullptr;
@@ -323,10 +338,12 @@ static void prune_layout(const ddg_index* idx, uint32_t Rsel, uint32_t want_ccap
   a.off_cand = 2 * row + round_up(Rsel * 4, 16);
   a.off_rows = fixed(ccap);
   // integer rows of at most 256 bytes: tasks of up to 256 candidates prune
-  // from an MFMA Gram matrix (its LDS replaces the row cache)
+  // from an MFMA Gram matrix (its LDS replaces the row cache).  Its tiles
+  // split each row into four lane groups of whole 16-byte K slabs, which
+  // covers the row only when it is a multiple of 64 bytes.
   a.gram_max = 0;
   a.off_gram = fixed(ccap);
-  if (idx->es == 1 && idx->ad <= 256 && prune_gram_enabled()) {
+  if (idx->es == 1 && idx->ad <= 256 && idx->ad % 64 == 0 && prune_gram_enabled()) {
     a.gram_max = std::min<uint32_t>(ccap, 256);
     a.rrows = 0;
     a.lds_per_wave = round_up(fixed(ccap) + prune_gram_lds_bytes(), 16);
@@ -425,6 +442,7 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
     t.stop();
     uint32_t ov = 0;
     HIPCHK(hipMemcpy(&ov, flags + 1, 4, hipMemcpyDeviceToHost));
+    if (ov & 2u) return fail(idx, DDG_ERANGE, "start point out of range");
     if (!ov) {
       if (cmps_total) {
         std::vector<uint64_t> c(B);
@@ -1146,6 +1164,25 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
                              const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
                              const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids, float* out_dists,
                              uint64_t* dist_cmps, uint32_t* visited) {
+  // every host-side scalar is checked before anything is launched: the
+  // kernel trusts nstarts (<= 64 seeds one lane each), the frontier capacity
+  // (beam_max) and cut >= 1 (the exactness of the visited flag)
+  if (!idx || !qp) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nq && (!queries || !out_ids || !starts)) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nstarts == 0 || nstarts > 64) return fail(idx, DDG_EINVAL, "nstarts must be in [1, 64]");
+  if (qp->trav != DDG_TRAV_NATIVE && qp->trav != DDG_TRAV_BF16 && qp->trav != DDG_TRAV_FP8E4M3)
+    return fail(idx, DDG_EINVAL, "unknown traversal mode");
+  if (qp->trav != DDG_TRAV_NATIVE && idx->dtype != DDG_F32)
+    return fail(idx, DDG_EINVAL, "reduced-precision traversal needs a float32 index");
+  if (qp->trav != DDG_TRAV_NATIVE && qp->rerank != 0 && qp->rerank < qp->k)
+    return fail(idx, DDG_EINVAL, "rerank must be 0 (all) or at least k");
+  if (beam_max == 0 || qp->k > beam_max)
+    return fail(idx, DDG_EINVAL, "Error: beam search parameter Q smaller than k");  // beamSearch.h:327-331
+  if (!per_query_beam && (qp->beam == 0 || qp->beam > beam_max || qp->k > qp->beam))
+    return fail(idx, DDG_EINVAL, "beam must be in [k, beam_max]");
+  if (qp->k > 0 && qp->cut < 1.0 && idx->metric == DDG_L2)
+    return fail(idx, DDG_EINVAL, "cut below 1.0 is not supported");
+  if (nq >= (1ull << 32)) return fail(idx, DDG_EINVAL, "nq must be below 2^32");
   SearchArgs a{};
   Launch l{};
   const int trav = (int)qp->trav;
@@ -1177,7 +1214,7 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   a.R = idx->R;
   a.nstarts = nstarts;
   a.k = qp->k;
-  a.beam = qp->beam;
+  a.beam = per_query_beam ? beam_max : qp->beam;  // per-query beams are clamped to beam_max
   a.degree_limit = qp->degree_limit ? std::min(qp->degree_limit, idx->R) : idx->R;
   a.build_mode = 0;
   a.cut_trim = idx->metric == DDG_L2 ? 1 : 0;
@@ -1188,6 +1225,9 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   Timer t(idx, &idx->prof[0]);
   HIPCHK(launch_search(idx->dtype, idx->metric, trav, a, l, idx->stream));
   t.stop();
+  uint32_t ov = 0;
+  HIPCHK(hipMemcpy(&ov, flags + 1, 4, hipMemcpyDeviceToHost));
+  if (ov & 2u) return fail(idx, DDG_ERANGE, "start point out of range");
   return DDG_OK;
 }
 

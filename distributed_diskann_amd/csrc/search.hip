@@ -138,7 +138,13 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 
     const uint32_t q = fetch_task(a.work);
     if (q >= a.nq) break;
 
-    const uint32_t beam = a.per_query_beam ? a.per_query_beam[q] : a.beam;
+    // a per-query beam is clamped to [1, beam]: the frontier and the filter
+    // were sized for the largest beam the host declared
+    uint32_t beam = a.beam;
+    if (a.per_query_beam) {
+      const uint32_t bq = a.per_query_beam[q];
+      beam = bq < 1u ? 1u : (bq < a.beam ? bq : a.beam);
+    }
     const uint32_t mask = (1u << filter_bits(beam)) - 1u;
     uint32_t self_id;  // ids and query indices are below 2^32
     const T* qsrc;
@@ -169,8 +175,14 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 
     wave_sync();
     const typename P::Q qs = P::prep(a, ql);
 
-    // starting frontier, sorted by (distance, id)
-    const uint32_t ns = a.nstarts;
+    // starting frontier, sorted by (distance, id).  A start id outside
+    // [0, n) (device memory the host could not check) raises bit 1 of the
+    // status word and the query runs with no start: an empty frontier
+    uint32_t ns = a.nstarts;
+    if (ballot(lane < ns && a.starts[lane] >= a.n)) {
+      if (lane == 0) atomicOr(a.overflow, 2u);
+      ns = 0;
+    }
     {
       uint2 se = make_uint2(0u, 0u);
       if (lane < ns) {

@@ -130,10 +130,21 @@ class Index:
         """Search over torch CUDA tensors (device-resident inputs and outputs)."""
         if starts is None:
             raise ValueError("search_device needs a device tensor of start points")
+        want = {"uint8": "torch.uint8", "int8": "torch.int8", "float32": "torch.float32"}[self.dtype]
+        if str(Q.dtype) != want or Q.dim() != 2 or Q.shape[1] != self.ad or not Q.is_contiguous():
+            raise ValueError(f"queries must be a contiguous (nq, {self.ad}) {want} tensor")
+        nq = Q.shape[0]
+        for name, t, size in (("out_ids", out_ids, nq * k), ("out_dists", out_dists, nq * k), ("cmps", cmps, nq),
+                              ("visited", visited, nq), ("per_query_beam", per_query_beam, nq),
+                              ("query_ids", query_ids, nq)):
+            if t is not None and (t.numel() < size or not t.is_contiguous() or t.element_size() not in (4, 8)):
+                raise ValueError(f"{name} must be a contiguous tensor of at least {size} elements")
+        if cmps is not None and cmps.element_size() != 8:
+            raise ValueError("cmps must hold 64-bit counters")
         qp = QueryParamsC(k, beam, cut, limit, degree_limit, TRAV[trav], rerank)
         p = lambda t: None if t is None else C.c_void_p(t.data_ptr())
         beam_max = beam if beam_max is None else beam_max
-        nst = 1 if starts is None else starts.numel()
+        nst = starts.numel()
         self._chk(lib().ddg_search_device(self.h, p(Q), Q.shape[0], p(query_ids), p(starts), nst, C.byref(qp),
                                           p(per_query_beam), beam_max, p(out_ids), p(out_dists), p(cmps),
                                           p(visited)))

@@ -128,9 +128,13 @@ ddg_status ddg_search(ddg_index* idx, const void* queries /* nq*aligned_d */, ui
 
 /* ddg_search over device memory: queries, query_ids, starts,
  * per_query_beam and every output are device pointers (e.g. torch tensors on
- * the same GPU); beam_max bounds per_query_beam (or equals qp->beam).  No
- * argument validation beyond the launch shape; the call returns once the
- * search kernel has finished. */
+ * the same GPU); beam_max bounds per_query_beam (or is at least qp->beam).
+ * The host scalars are checked as ddg_search checks them (nstarts in
+ * [1, 64], 0 < k <= beam <= beam_max, cut >= 1 on L2 when k > 0, the
+ * traversal mode against the dtype) before any launch.  Device contents the
+ * host cannot see: a per-query beam is clamped to [1, beam_max], and a start
+ * id >= n fails the call with DDG_ERANGE (that query's outputs are then
+ * empty).  The call returns once the search kernel has finished. */
 ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
                              const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
                              const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids, float* out_dists,

@@ -9,7 +9,7 @@
 // no -march, plus -ffp-contract=off, so the FP32 summation order of
 // NSGDist.h:113-131 and libstdc++'s std::sort tie order are the reference's.
 //
-// Function → reference anchor → what pins it (tests/test_oracle_*.py):
+// Function → reference anchor → what pins it (tests/test_oracle*.py):
 //   dist_*            euclidian_point.h:44-65, NSGDist.h:113-131,
 //                     mips_point.h:44-66          golden vectors from oracle/_ref
 //                                                 (real reference headers), int64
@@ -33,11 +33,17 @@
 //                     utilities.h:131-140         random_permutation) at n from 0
 //                     (hash64), counting_sort.h   to 1M, both sides of the
 //                     (stable bucket sort)        sequential threshold
-//   batch_insert /    index.h:125-137, 213-341;   recall on random-xs against the
-//   build             bsp_index.hpp:141-434 (P=1)  reference's own GT file; graph
-//                                                 invariants (degree <= R, no self
-//                                                 loops, ids < n); BSP(P=1) ==
-//                                                 build_index without the final sort
+//   batch_insert /    index.h:125-137, 213-341;   closed-form builds (hand-worked
+//   build             bsp_index.hpp:141-434 (P=1)  1-D cases at |S|+|G[t]| = R and
+//                                                 R+1, a reverse prune that keeps
+//                                                 old edges, a two_pass build; huge
+//                                                 alpha and alpha = 0 closed forms),
+//                                                 each of three planted mistakes
+//                                                 caught (tests/test_oracle_mutants.py);
+//                                                 recall on random-xs against the
+//                                                 reference's own GT file; graph
+//                                                 invariants; BSP(P=1) == build_index
+//                                                 without the final sort
 //   ground_truth      (sklearn brute force in the  numpy float64 brute force
 //                     reference's dataset tools)
 //

@@ -9,6 +9,7 @@ imports it.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -52,36 +53,55 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        _lib = C.CDLL(LIB_PATH)
-        P = C.POINTER
-        _lib.orc_hash64_2.restype = C.c_uint64
-        _lib.orc_hash64_2.argtypes = [C.c_uint64]
-        _lib.orc_random_permutation.argtypes = [C.c_uint64, C.c_void_p]
-        _lib.orc_random_permutation.restype = None
-        _lib.orc_beam_search.argtypes = [P(_Points), P(_Graph), C.c_void_p, C.c_int64, C.c_void_p, C.c_uint32,
-                                         C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int64,
-                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
-                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
-        _lib.orc_search_all.argtypes = [P(_Points), P(_Graph), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
-                                        C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int64, C.c_void_p,
-                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
-        _lib.orc_robust_prune.argtypes = [P(_Points), P(_Graph), C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
-                                          C.c_double, C.c_int, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p]
-        _lib.orc_schedule.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_uint64, C.c_void_p, C.c_void_p,
-                                      C.c_uint64, C.c_void_p]
-        _lib.orc_group_by_key.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
-                                          C.c_void_p, C.c_void_p]
-        _lib.orc_batch_insert.argtypes = [P(_Points), P(_Graph), C.c_void_p, C.c_uint64, C.c_double, C.c_double,
-                                          C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
-                                          C.c_void_p, C.c_void_p]
-        _lib.orc_build.argtypes = [P(_Points), P(_Graph), P(_BuildParams), C.c_int, C.c_int, C.c_void_p,
-                                   C.c_void_p]
-        _lib.orc_sort_rows.argtypes = [P(_Points), P(_Graph), C.c_int, C.c_int]
-        _lib.orc_ground_truth.argtypes = [P(_Points), C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p,
-                                          C.c_int]
-        _lib.orc_distances.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
-                                       C.c_uint32, C.c_void_p]
+        _lib = load(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Runs the wrapper's functions against another build of oracle.cpp (the
+    mutation tests compile deliberately broken copies)."""
+    global _lib
+    saved = lib()
+    _lib = load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def load(path):
+    """A ctypes handle on a build of oracle.cpp with every signature set."""
+    L = C.CDLL(path)
+    P = C.POINTER
+    L.orc_hash64_2.restype = C.c_uint64
+    L.orc_hash64_2.argtypes = [C.c_uint64]
+    L.orc_random_permutation.argtypes = [C.c_uint64, C.c_void_p]
+    L.orc_random_permutation.restype = None
+    L.orc_beam_search.argtypes = [P(_Points), P(_Graph), C.c_void_p, C.c_int64, C.c_void_p, C.c_uint32,
+                                     C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int64,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]
+    L.orc_search_all.argtypes = [P(_Points), P(_Graph), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
+                                    C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int64, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    L.orc_robust_prune.argtypes = [P(_Points), P(_Graph), C.c_uint32, C.c_void_p, C.c_void_p, C.c_uint32,
+                                      C.c_double, C.c_int, C.c_uint32, C.c_int, C.c_void_p, C.c_void_p]
+    L.orc_schedule.argtypes = [C.c_uint64, C.c_double, C.c_double, C.c_uint64, C.c_void_p, C.c_void_p,
+                                  C.c_uint64, C.c_void_p]
+    L.orc_group_by_key.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
+    L.orc_batch_insert.argtypes = [P(_Points), P(_Graph), C.c_void_p, C.c_uint64, C.c_double, C.c_double,
+                                      C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_int,
+                                      C.c_void_p, C.c_void_p]
+    L.orc_build.argtypes = [P(_Points), P(_Graph), P(_BuildParams), C.c_int, C.c_int, C.c_void_p,
+                               C.c_void_p]
+    L.orc_sort_rows.argtypes = [P(_Points), P(_Graph), C.c_int, C.c_int]
+    L.orc_ground_truth.argtypes = [P(_Points), C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p,
+                                      C.c_int]
+    L.orc_distances.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
+                                   C.c_uint32, C.c_void_p]
+    return L
 
 
 def _ptr(a):
