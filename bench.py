@@ -1,28 +1,33 @@
 """Benchmark: search QPS at 10-recall@10 >= 0.95 and index build seconds on
-one MI355X.
+one MI355X, reduced-precision traversal against FP32.
 
-Main line (`value`): the 100,000 x 20 float32 workload (seed 2 base, seed 3
-queries; R=32, L=64, alpha=1.2; k=10).  A step is one batched search of the
-10,000 device-resident queries at the smallest uniform beam width whose
-10-recall@10 (FP64 brute-force ground truth, ties counted as in
-check_nn_recall.h:86-109) is >= 0.95.  The build is timed once (device time,
-vectors resident), and the per-query-beam launch of the workload (beams
-uniform over {10, 20, 50, 100, 200}) is timed too.
+Headline workload: 10,000,000 x 200 float32, maximum inner product, a
+5,000-component Gaussian mixture (centres N(0,1), scale 0.3, seed 8) with
+10,000 queries from shifted centres (seed 9); Vamana R=64, L=128,
+alpha=1.0; k=10.  The build is timed once (device time, vectors resident).
+A step is one batched search launch over the 10,000 device-resident
+queries.  For each traversal mode -- FP32 (the reference's arithmetic), BF16
+and per-dimension-scaled FP8-e4m3 copies with FP32 accumulation and an FP32
+re-rank of the final 100 -- the smallest beam of the reference's sweep list
+(check_nn_recall.h:173-175) whose 10-recall@10 against FP64 brute-force
+ground truth (K = 100, ties at the 10th distance counted as
+check_nn_recall.h:86-109 counts them) is >= 0.95 is found, then W warm-up and
+K timed steps run.  `value` is the QPS of the fastest reduced-precision mode;
+every mode's QPS, beam, recall, kernel time and distance error against FP64
+are in `modes`.
 
-`low_precision`: the maximum-inner-product mixture workload (5,000 centres
-N(0,1), scale 0.3, seed 8; queries from shifted centres, seed 9; d=200,
-R=64, L=128, alpha=1.0), at --lowp-n base points: QPS at recall 0.95 for
-FP32, BF16 and FP8-e4m3 traversal (re-rank of the final 100 in FP32), with
-each mode's distance error against FP64.
+`configs` times the other workloads the same way (build seconds, QPS at
+recall 0.95): 100,000 x 20 float32 L2 (with its per-query-beam launch),
+1,000,000 x 100 int8 and 10,000,000 x 128 uint8.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--lowp-n N]
-    python bench.py --workload {i8,u8,mips} [--n N] [--nq NQ] [--lowp-n 0]
+`cpu_baseline` is the oracle (a CPU restatement of the reference's build and
+search, g++ -O2 like its Makefile) on every core this process may use: the
+FP32 search of all queries at the FP32 beam over the GPU-built graph (the
+same graph gives the same visits), plus a build of the first --cpu-build-n
+points as a sample of the build rate.  A full 10M-point CPU build does not
+fit this run and is not extrapolated.
 
-Other workloads run the same protocol (build, FP64 ground truth, Ls sweep,
-timed device-resident search; float32 workloads also time BF16 and FP8
-traversal with FP32 re-rank).  Above --cpu-full-max points the CPU baseline
-is a bounded sample: the oracle builds the first --cpu-build-n points, and
-searches --cpu-nq queries over the GPU-built graph at the chosen Ls.
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-configs]
 
 Prints one JSON line.
 """
@@ -31,6 +36,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -43,7 +49,9 @@ METRIC = "search QPS at 10-recall@10=0.95 + index build sec, 1 MI355X, low-prec 
 HBM_PEAK = 8.0e12
 BEAMS = [10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 45, 50, 55, 60, 65,
          70, 80, 90, 100, 120, 140, 160, 180, 200, 225, 250, 275, 300, 375, 500, 750, 1000]  # check_nn_recall.h:173-175
-ESIZE = {"native": None, "bf16": 2, "fp8": 1}
+S_TRAV = {"bf16": 2, "fp8": 1}
+GT_K = 100
+RERANK = 100
 
 
 def log(*a):
@@ -69,7 +77,9 @@ class DeviceSearch:
                                starts=self.starts, per_query_beam=pqb, beam_max=bmax, trav=trav, rerank=rerank)
 
     def timed(self, Ls, steps, warmup, trav="native", rerank=0, pqb=None, bmax=None):
-        """(wall ms per step, kernel ms per step) over `steps` launches."""
+        """(wall ms per step, kernel ms per step) over `steps` launches; the
+        kernel time is the search kernel's own, from HIP events recorded on
+        the stream it runs on."""
         for _ in range(warmup):
             self.run(Ls, trav, rerank, pqb, bmax)
         self.torch.cuda.synchronize()
@@ -92,12 +102,13 @@ class DeviceSearch:
 def alg_bytes(cmps, vis, nq, d, s_trav, avg_deg, rerank_rows=0):
     """Algorithmic bytes of one search launch: every distance reads one
     d-element row of the traversed copy, every expansion one adjacency row
-    (1 + degree words), every query its FP32/native row, and a re-rank reads
-    rerank_rows full FP32 rows per query."""
+    (1 + degree words), every query its row, and a re-rank reads rerank_rows
+    full FP32 rows per query."""
     return cmps * d * s_trav + vis * 4.0 * (1.0 + avg_deg) + nq * d * s_trav + nq * rerank_rows * d * 4.0
 
 
 def sweep(ds, gt_ids, gt_d, trav="native", rerank=0, target=0.95):
+    """Smallest beam of the reference's list reaching the target recall."""
     from distributed_diskann_amd.index import recall_with_ties
     last = None
     for Ls in BEAMS:
@@ -125,57 +136,145 @@ def distance_error(idx, X, Q, d, metric, mode, nq=100, m=1000, seed=17):
     return {"median": float(np.median(err)), "p99": float(np.quantile(err, 0.99)), "max": float(err.max())}
 
 
-def profiled_traffic(workload, n, Ls):
-    """HBM bytes per search launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE
-    (copy-calibrated) for this exact workload and beam, from the committed
-    profile summary; None when no profile covers it."""
-    covered = {("s", 100_000, 80): "k_search_f32_100Kx20_Ls80.md",
-               ("u8", 10_000_000, 140): "k_search_u8_10Mx128_Ls140.md"}
-    name = covered.get((workload, n, Ls))
-    if name is None or not os.path.exists(os.path.join(HERE, "profiles", name)):
+def profiled_traffic(key):
+    """HBM bytes per search launch measured by rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (copy-calibrated) for this exact workload, mode and beam, from
+    a committed profile summary naming that key; (None, None) otherwise."""
+    pdir = os.path.join(HERE, "profiles")
+    if not os.path.isdir(pdir):
         return None, None
-    for line in open(os.path.join(HERE, "profiles", name)):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1]), "profiles/" + name
+    for name in sorted(os.listdir(pdir)):
+        path = os.path.join(pdir, name)
+        key_ok, val = False, None
+        for line in open(path):
+            if line.startswith("bench_key:") and line.split(":", 1)[1].strip() == key:
+                key_ok = True
+            if line.startswith("traffic_bytes_per_launch:"):
+                val = float(line.split(":")[1])
+        if key_ok and val is not None:
+            return val, "profiles/" + name
     return None, None
 
 
-def low_precision(torch, args):
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None  # a cgroup CPU quota caps the cores a process can use below its affinity mask
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return {"model": model, "nproc": os.cpu_count(), "affinity_cores": affinity, "cgroup_cpu_quota": quota,
+            "usable_cores": usable}
+
+
+def measure_mode(ds, idx, X, Q, w, gi, gd, mode, steps, warmup, avg_deg, key):
+    """Sweep, autotune and time one traversal mode of an index."""
+    from distributed_diskann_amd.index import recall_with_ties
+    rerank = 0 if mode == "native" else RERANK
+    prep = idx.prepare_traversal(mode)
+    Ls, r_sweep = sweep(ds, gi, gd, mode, rerank)
+    tune = idx.autotune_search(Q, 10, Ls, trav=mode, rerank=rerank)
+    wall, kern = ds.timed(Ls, steps, warmup, mode, rerank)
+    rec = recall_with_ties(ds.result_ids(), gi, gd, 10)
+    cm, vs = ds.counters()
+    s_trav = (4 if w.dtype == "float32" else 1) if mode == "native" else S_TRAV[mode]
+    nr = 0 if mode == "native" else min(RERANK, Ls)
+    B = alg_bytes(cm, vs, ds.nq, w.d, s_trav, avg_deg, nr)
+    out = {"Ls": Ls, "recall": rec, "qps": ds.nq / (wall / 1000.0), "ms_per_step": wall, "kernel_ms": kern,
+           "alg_bytes_per_launch": B, "alg_GBps": B / (kern / 1000.0) / 1e9,
+           "hbm_frac": B / (kern / 1000.0) / HBM_PEAK, "dist_cmps_per_query": cm / ds.nq,
+           "visited_per_query": vs / ds.nq, "prepare_s": prep, "autotune": tune}
+    if w.dtype == "float32":
+        out["dist_rel_err_vs_fp64"] = distance_error(idx, X, Q, w.d, w.metric, mode)
+    log(f"[bench] {key} {mode}: Ls={Ls} recall {rec:.4f} QPS {out['qps']:.0f} kernel {kern:.3f} ms "
+        f"({out['alg_GBps']:.0f} GB/s algorithmic, {out['hbm_frac']:.3f} of HBM)")
+    return out
+
+
+def run_workload(torch, name, modes, steps, warmup, extra=None):
+    """Generates, builds and times one workload; returns (result, X, Q, idx, graph)."""
     from distributed_diskann_amd import Index, datasets
-    X, Q, w = datasets.generate("mips", n=args.lowp_n, nq=args.lowp_nq)
-    nq = Q.shape[0]
-    log(f"[bench] low-precision workload: n={X.shape[0]} d={w.d} mips R={w.R} L={w.L} alpha={w.alpha} nq={nq}")
-    idx = Index(X, w.d, "mips", w.R)
+    t0 = time.perf_counter()
+    X, Q, w = datasets.generate(name)
+    gen_s = time.perf_counter() - t0
+    log(f"[bench] {name}: {X.shape[0]}x{w.d} {w.dtype} {w.metric} R={w.R} L={w.L} alpha={w.alpha} "
+        f"nq={Q.shape[0]} (generated in {gen_s:.1f} s)")
+    idx = Index(X, w.d, w.metric, w.R)
     build_s = idx.build(R=w.R, L=w.L, alpha=w.alpha)
+    bprof = {k: v for k, v in idx.profile().items() if k.endswith("_ms")}
     G = idx.get_graph()
     avg_deg = float(G[:, 0].mean())
-    del G
-    gi, gd = idx.ground_truth(Q, 10)
+    log(f"[bench] {name}: build {build_s:.3f} s (device), average degree {avg_deg:.2f}")
+    gi, gd = idx.ground_truth(Q, GT_K)
     gt_ms = idx.profile()["dense_ms"]
-    log(f"[bench] lowp build {build_s:.2f} s, avg degree {avg_deg:.2f}, FP64 GT {gt_ms:.0f} ms")
+    log(f"[bench] {name}: FP64 ground truth top-{GT_K} {gt_ms:.1f} ms")
     ds = DeviceSearch(torch, idx, Q)
-    out = {"workload": f"{X.shape[0]}x{w.d} f32 MIPS mixture, R={w.R} L={w.L} alpha={w.alpha}, k=10, nq={nq}",
-           "build_seconds": build_s, "gt_ms": gt_ms, "modes": {}}
-    for mode in ("native", "bf16", "fp8"):
-        rerank = 0 if mode == "native" else 100
-        prep = idx.prepare_traversal(mode)
-        Ls, rec = sweep(ds, gi, gd, mode, rerank)
-        idx.autotune_search(Q, 10, Ls, trav=mode, rerank=rerank)
-        wall, kern = ds.timed(Ls, max(3, args.steps // 4), 1, mode, rerank)
-        cm, vs = ds.counters()
-        s_trav = 4 if mode == "native" else ESIZE[mode]
-        nr = 0 if mode == "native" else min(100, Ls)
-        B = alg_bytes(cm, vs, nq, w.d, s_trav, avg_deg, nr)
-        err = distance_error(idx, X, Q, w.d, "mips", mode) if mode != "native" else None
-        out["modes"][mode] = {"Ls": Ls, "recall": rec, "qps": nq / (wall / 1000.0), "kernel_ms": kern,
-                              "wall_ms": wall, "alg_GBps": B / (kern / 1000.0) / 1e9,
-                              "hbm_frac": B / (kern / 1000.0) / HBM_PEAK, "prepare_s": prep,
-                              "dist_rel_err_vs_fp64": err}
-        log(f"[bench] lowp {mode}: Ls={Ls} recall {rec:.4f} QPS {nq / (wall / 1000.0):.0f} kernel {kern:.3f} ms "
-            f"{B / (kern / 1000.0) / 1e9:.0f} GB/s err {err}")
-    del ds, idx
-    torch.cuda.empty_cache()
-    return out
+    res = {"workload": f"{X.shape[0]}x{w.d} {w.dtype} {w.metric}, R={w.R} L={w.L} alpha={w.alpha}, "
+                       f"k=10, {Q.shape[0]} queries", "build_seconds": build_s, "build_profile_ms": bprof,
+           "avg_degree": avg_deg, "ground_truth_ms": gt_ms, "ground_truth_k": GT_K, "modes": {}}
+    for mode in modes:
+        res["modes"][mode] = measure_mode(ds, idx, X, Q, w, gi, gd, mode, steps, warmup, avg_deg, name)
+    if extra:
+        extra(ds, idx, Q, gi, gd, res)
+    del ds
+    return res, X, Q, w, idx, G, gi, gd
+
+
+def per_query_beams(torch, steps):
+    def extra(ds, idx, Q, gi, gd, res):
+        from distributed_diskann_amd import datasets
+        from distributed_diskann_amd.index import recall_with_ties
+        beams = datasets.s_query_beams(Q.shape[0])
+        pqb = torch.from_numpy(beams.astype(np.int32)).to(torch.device("cuda:0"))
+        idx.autotune_search(Q, 10, int(beams.max()))
+        wall, kern = ds.timed(int(beams.max()), steps, 1, pqb=pqb, bmax=int(beams.max()))
+        res["per_query_beam"] = {"qps": Q.shape[0] / (wall / 1000.0), "kernel_ms": kern,
+                                 "recall": recall_with_ties(ds.result_ids(), gi, gd, 10),
+                                 "beams": [int(b) for b in np.unique(beams)]}
+        log(f"[bench] per-query beams: {res['per_query_beam']}")
+    return extra
+
+
+def cpu_baseline(X, Q, w, G, gi, gd, Ls, build_n):
+    """The oracle on every usable core: FP32 search of all queries at beam Ls
+    over the GPU-built graph, and a build of the first build_n points."""
+    from distributed_diskann_amd.index import recall_with_ties
+    from oracle import oracle as O
+    info = cpu_info()
+    threads = info["usable_cores"]
+    m = min(build_n, X.shape[0])
+    Ps = O.Points(np.ascontiguousarray(X[:m]), w.d, w.metric)
+    t0 = time.perf_counter()
+    O.build(Ps, w.R, w.L, w.alpha, tie_mode=O.TIE_REFERENCE, nthreads=threads)
+    build_s = time.perf_counter() - t0
+    del Ps
+    P = O.Points(X, w.d, w.metric)
+    Gg = O.Graph(G)
+    O.search_all(P, Gg, Q[:200], 10, Ls, cut=1.35, nthreads=threads)  # warm-up (check_nn_recall.h:62-66)
+    t0 = time.perf_counter()
+    ci, _, _, _ = O.search_all(P, Gg, Q, 10, Ls, cut=1.35, nthreads=threads)
+    search_s = time.perf_counter() - t0
+    rec = recall_with_ties(ci, gi, gd, 10)
+    log(f"[bench] CPU oracle ({threads} threads, {info['model']}): build of {m} points {build_s:.2f} s, "
+        f"FP32 search {Q.shape[0] / search_s:.0f} QPS at Ls={Ls} (recall {rec:.4f})")
+    return {"value": Q.shape[0] / search_s, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 search of all {Q.shape[0]} queries at Ls={Ls} over the GPU-built "
+                      f"{X.shape[0]}-point graph (recall {rec:.4f}); build of the first {m} points in "
+                      f"{build_s:.2f} s ({m / build_s:.0f} points/s); the full {X.shape[0]}-point CPU build "
+                      "does not fit this run and is not extrapolated",
+            "search_seconds": search_s, "build_points_sample": m, "build_seconds_sample": build_s,
+            "cpu": info}
 
 
 def main():
@@ -183,24 +282,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="s")
-    ap.add_argument("--n", type=int, default=0, help="base points (0 = the workload's own size)")
-    ap.add_argument("--nq", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-full-max", type=int, default=200_000)
-    ap.add_argument("--cpu-build-n", type=int, default=100_000)
-    ap.add_argument("--cpu-nq", type=int, default=1000)
-    ap.add_argument("--lowp-n", type=int, default=1_000_000, help="base points of the low-precision workload "
-                                                                  "(0 skips it)")
-    ap.add_argument("--lowp-nq", type=int, default=10_000)
+    ap.add_argument("--no-configs", action="store_true", help="headline workload only")
+    ap.add_argument("--modes", default="native,bf16,fp8")
+    ap.add_argument("--cpu-build-n", type=int, default=200_000)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the FETCH_SIZE / "
                          "WRITE_SIZE counters against known bytes) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the multi-GPU path is "
-              "design-only)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: this build runs on one "
+              "GPU, and its multi-GPU path is a design, not code", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -215,132 +307,57 @@ def main():
         print(json.dumps({"copy_probe_bytes": nbytes, "launches": 5}), flush=True)
         return
 
-    from distributed_diskann_amd import Index, datasets
-    from distributed_diskann_amd.index import recall_with_ties
-
-    t0 = time.perf_counter()
-    X, Q, w = datasets.generate(args.workload, n=args.n or None, nq=args.nq or None)
-    n, nq = X.shape[0], Q.shape[0]
-    log(f"[bench] workload {w.name}: n={n} d={w.d} {w.dtype} {w.metric} R={w.R} L={w.L} alpha={w.alpha} nq={nq} "
-        f"(generated in {time.perf_counter() - t0:.1f} s)")
-    idx = Index(X, w.d, w.metric, w.R)
-    t0 = time.perf_counter()
-    build_s = idx.build(R=w.R, L=w.L, alpha=w.alpha)
-    build_wall = time.perf_counter() - t0
-    bprof = idx.profile()
-    G = idx.get_graph()
-    avg_deg = float(G[:, 0].mean())
-    log(f"[bench] build {build_s:.3f} s device ({build_wall:.3f} s wall); avg degree {avg_deg:.2f}")
-
-    gt_ids, gt_d = idx.ground_truth(Q, 10)
-    gt_ms = idx.profile()["dense_ms"]
-    log(f"[bench] FP64 ground truth {gt_ms:.1f} ms")
-
-    ds = DeviceSearch(torch, idx, Q)
-    chosen, rec0 = sweep(ds, gt_ids, gt_d)
-    log(f"[bench] sweep -> Ls={chosen} (recall {rec0:.4f})")
-    tune = idx.autotune_search(Q, 10, chosen)
-    log(f"[bench] autotune at Ls={chosen}: {tune}")
-    ms_per_step, kern_avg_ms = ds.timed(chosen, args.steps, args.warmup)
-    qps = nq / (ms_per_step / 1000.0)
-    rec = recall_with_ties(ds.result_ids(), gt_ids, gt_d, 10)
-    tot_cmps, tot_vis = ds.counters()
-    s = 4 if w.dtype == "float32" else 1
-    achieved = alg_bytes(tot_cmps, tot_vis, nq, w.d, s, avg_deg) / (kern_avg_ms / 1000.0)
-    log(f"[bench] Ls={chosen}: {ms_per_step:.3f} ms/step wall, kernel {kern_avg_ms:.3f} ms, QPS {qps:.0f}, "
-        f"recall {rec:.4f}, {achieved / 1e9:.1f} GB/s algorithmic")
-
-    # the per-query-beam launch of the workload
-    beams = datasets.s_query_beams(nq)
-    pqb = torch.from_numpy(beams.astype(np.int32)).to(torch.device("cuda:0"))
-    idx.autotune_search(Q, 10, int(beams.max()))  # the launch shape of the largest beam class
-    mixed_ms, mixed_kern = ds.timed(chosen, max(1, args.steps // 4), 1, pqb=pqb, bmax=int(beams.max()))
-    mixed_rec = recall_with_ties(ds.result_ids(), gt_ids, gt_d, 10)
-    log(f"[bench] per-query beams: {mixed_ms:.3f} ms/step, recall {mixed_rec:.4f}")
-
-    # reduced-precision traversal of the same index (float32 workloads)
-    modes = None
-    if w.dtype == "float32":
-        modes = {}
-        for mode in ("bf16", "fp8"):
-            prep = idx.prepare_traversal(mode)
-            Ls_m, r_m = sweep(ds, gt_ids, gt_d, mode, 100)
-            idx.autotune_search(Q, 10, Ls_m, trav=mode, rerank=100)
-            wall_m, kern_m = ds.timed(Ls_m, max(3, args.steps // 4), 1, mode, 100)
-            cm_m, vs_m = ds.counters()
-            B_m = alg_bytes(cm_m, vs_m, nq, w.d, ESIZE[mode], avg_deg, min(100, Ls_m))
-            modes[mode] = {"Ls": Ls_m, "recall": r_m, "qps": nq / (wall_m / 1000.0), "kernel_ms": kern_m,
-                           "alg_GBps": B_m / (kern_m / 1000.0) / 1e9, "prepare_s": prep}
-            log(f"[bench] {mode}: Ls={Ls_m} recall {r_m:.4f} QPS {nq / (wall_m / 1000.0):.0f} kernel {kern_m:.3f} ms")
-    del ds
-
-    lowp = low_precision(torch, args) if (args.lowp_n > 0 and args.workload == "s") else None
+    modes = args.modes.split(",")
+    head, X, Q, w, idx, G, gi, gd = run_workload(torch, "mips", modes, args.steps, args.warmup)
+    low = {m: r for m, r in head["modes"].items() if m != "native" and r["recall"] >= 0.95}
+    best = max(low, key=lambda m: low[m]["qps"]) if low else "native"
+    top = head["modes"][best]
+    fp32 = head["modes"].get("native")
+    idx.close()
+    del idx
+    torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        if n <= args.cpu_full_max:
-            P = O.Points(X, w.d, w.metric)
-            t0 = time.perf_counter()
-            Gc = O.build(P, w.R, w.L, w.alpha, tie_mode=O.TIE_REFERENCE, nthreads=threads)
-            cpu_build = time.perf_counter() - t0
-            O.search_all(P, Gc, Q, 10, chosen, cut=1.35, nthreads=threads)  # warm-up (check_nn_recall.h:62-66)
-            t0 = time.perf_counter()
-            ci, _, _, _ = O.search_all(P, Gc, Q, 10, chosen, cut=1.35, nthreads=threads)
-            cpu_search = time.perf_counter() - t0
-            cpu_rec = recall_with_ties(ci, gt_ids, gt_d, 10)
-            cpu = {"value": nq / cpu_search, "unit": "queries/s", "cores": threads, "kind": "port",
-                   "sample": f"full workload: build of all {n} points and {nq} queries at Ls={chosen} "
-                             f"(CPU recall {cpu_rec:.4f})",
-                   "build_seconds": cpu_build}
-            log(f"[bench] CPU ({threads} threads): build {cpu_build:.2f} s, QPS {nq / cpu_search:.0f}")
-        else:
-            # bounded sample: a build of the first cpu_build_n points, and cpu_nq
-            # queries searched over the GPU-built graph (the same graph, so the
-            # same visits) at the chosen Ls
-            m = min(args.cpu_build_n, n)
-            Ps = O.Points(np.ascontiguousarray(X[:m]), w.d, w.metric)
-            t0 = time.perf_counter()
-            O.build(Ps, w.R, w.L, w.alpha, tie_mode=O.TIE_REFERENCE, nthreads=threads)
-            cpu_build_m = time.perf_counter() - t0
-            del Ps
-            P = O.Points(X, w.d, w.metric)
-            Gg = O.Graph(G)
-            qs = np.ascontiguousarray(Q[:args.cpu_nq])
-            O.search_all(P, Gg, qs[:100], 10, chosen, cut=1.35, nthreads=threads)  # warm-up
-            t0 = time.perf_counter()
-            O.search_all(P, Gg, qs, 10, chosen, cut=1.35, nthreads=threads)
-            cpu_search = time.perf_counter() - t0
-            cpu = {"value": qs.shape[0] / cpu_search, "unit": "queries/s", "cores": threads, "kind": "port",
-                   "sample": f"{qs.shape[0]} of {nq} queries at Ls={chosen} over the GPU-built {n}-point graph; "
-                             f"build of the first {m} points only",
-                   "build_seconds_sample": cpu_build_m, "build_points_per_s_sample": m / cpu_build_m}
-            log(f"[bench] CPU ({threads} threads): build of {m} points {cpu_build_m:.2f} s "
-                f"({m / cpu_build_m:.0f} points/s), QPS {qs.shape[0] / cpu_search:.0f}")
-            del P, Gg
+        cpu = cpu_baseline(X, Q, w, G, gi, gd, fp32["Ls"] if fp32 else top["Ls"], args.cpu_build_n)
+    del X, G
 
-    traffic, traffic_src = profiled_traffic(args.workload, n, chosen)
+    configs = {}
+    if not args.no_configs:
+        for name, extra in (("s", per_query_beams(torch, max(1, args.steps // 4))), ("i8", None), ("u8", None)):
+            r, *rest = run_workload(torch, name, ["native"], max(1, args.steps // 2), 1, extra)
+            rest[3].close()
+            del rest
+            torch.cuda.empty_cache()
+            configs[{"s": "100Kx20_f32", "i8": "1Mx100_i8", "u8": "10Mx128_u8"}[name]] = r
+
+    key = f"mips {best} Ls={top['Ls']}"
+    traffic, traffic_src = profiled_traffic(key)
+    kname = f"k_search<float,mips,{best}>"
     out = {
-        "metric": METRIC, "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": {"float32": "f32", "uint8": "u8", "int8": "i8"}[w.dtype],
-        "data": f"synthetic {w.name}: {n}x{w.d} {w.dtype} seed {w.base_seed}, {nq} queries seed {w.query_seed}",
-        "config": {"workload": f"{n}x{w.d} {w.dtype} {w.metric}, R={w.R} L={w.L} alpha={w.alpha}, k=10, Ls={chosen}",
-                   "n": n, "d": w.d, "nq": nq, "R": w.R, "L": w.L, "alpha": w.alpha, "k": 10, "Ls": chosen},
-        "recall": rec,
-        "autotune": tune,
-        "build_seconds": build_s,
-        "build_profile_ms": {k: v for k, v in bprof.items() if k.endswith("_ms")},
-        "per_query_beam": {"qps": nq / (mixed_ms / 1000.0), "kernel_ms": mixed_kern, "recall": mixed_rec,
-                           "beams": [10, 20, 50, 100, 200]},
-        "reduced_traversal": modes,
-        "low_precision": lowp,
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": f"k_search<{w.dtype},{w.metric},native>", "kernel_avg_ms": kern_avg_ms,
-                     "note": (f"{X.nbytes / 1e6:.1f} MB of vectors" +
-                              (": the working set is cache-resident" if X.nbytes < 256e6 else ""))},
+        "metric": METRIC, "value": top["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": top["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if best == "native" else f"{'bf16' if best == 'bf16' else 'fp8e4m3'} rows, f32 accumulate, "
+                                                "f32 re-rank",
+        "data": f"synthetic: {head['workload']}; 5000-component mixture seed {w.base_seed}, shifted queries seed "
+                f"{w.query_seed}",
+        "config": {"workload": f"10Mx200 f32 MIPS mixture, R=64 L=128 alpha=1.0, k=10, 10000 queries, "
+                               f"{best} traversal + FP32 re-rank of {RERANK}, Ls={top['Ls']}",
+                   "n": 10_000_000, "d": 200, "nq": 10_000, "R": 64, "L": 128, "alpha": 1.0, "k": 10,
+                   "mode": best, "Ls": top["Ls"]},
+        "recall": top["recall"],
+        "build_seconds": head["build_seconds"],
+        "fp32": None if fp32 is None else {"qps": fp32["qps"], "Ls": fp32["Ls"], "recall": fp32["recall"]},
+        "speedup_vs_fp32": None if fp32 is None else top["qps"] / fp32["qps"],
+        "headline": head,
+        "configs": configs,
+        "roofline": {"bound": "hbm", "achieved": top["alg_GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": top["hbm_frac"], "traffic": traffic, "traffic_source": traffic_src,
+                     "kernel": kname, "kernel_avg_ms": top["kernel_ms"],
+                     "alg_bytes_per_launch": top["alg_bytes_per_launch"],
+                     "note": "10M x 200 rows: the traversed copy and the 2.6 GB graph are far above the 256 MB "
+                             "Infinity Cache"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -12,7 +12,7 @@
 
 namespace ddg {
 
-constexpr uint32_t kGtMaxK = 32;
+constexpr uint32_t kGtMaxK = 100;  // the reference's ground-truth files hold 100 neighbours
 
 template <typename T, int METRIC>
 __device__ __forceinline__ double exact_dist(const T* row, const T* q, uint32_t d) {
@@ -121,7 +121,10 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr uint32_t kGtRows = 16;    // queries per wave
 constexpr uint32_t kGtTile = 32;    // points per tile
 constexpr uint32_t kGtWavesF = 4;   // waves per block, FP64 path
-constexpr uint32_t kGtWavesI8 = 8;  // waves per block, int8 path: 128 queries share each tile
+constexpr uint32_t kGtWavesI8 = 8;  // waves per block, int8 path: 128 queries share each tile (k <= 32)
+constexpr uint32_t kGtWavesI8Big = 4;  // k > 32: longer candidate lists, 64 queries per tile
+constexpr uint32_t kGtSlabs = 3;   // candidate-list entries per lane in a compaction (cap <= 192)
+static_assert(kGtTile + (kGtMaxK + 15) / 16 * 16 <= kWave * kGtSlabs, "a candidate list must fit one compaction");
 constexpr uint32_t kGtMaxQf = 64;   // FP64 path: dims per lane group (d <= 256)
 constexpr uint32_t kGtMaxSteps = 8; // int8 path: K steps of 64 (d <= 512)
 
@@ -173,26 +176,34 @@ struct GtTopk {
       ci[row * cap + pos] = id;
     }
   }
-  // keeps the k smallest of row's list, sorted; the k-th becomes the threshold
+  // keeps the k smallest of row's list, sorted; the k-th becomes the threshold.
+  // A list holds at most cap <= kWave * kGtSlabs entries, kGtSlabs per lane;
+  // its ids are distinct (a tile offers each point once per query), so the
+  // ranks are a permutation.
   __device__ void compact(uint32_t row) {
     const uint32_t l = lane_id();
     const uint32_t c = cnt[row];
     double* D = cd + row * cap;
     uint32_t* I = ci + row * cap;
-    double d0 = 0.0, d1 = 0.0;
-    uint32_t i0 = 0, i1 = 0, r0 = 0, r1 = 0;
-    const bool h0 = l < c, h1 = l + kWave < c;
-    if (h0) { d0 = D[l]; i0 = I[l]; }
-    if (h1) { d1 = D[l + kWave]; i1 = I[l + kWave]; }
+    double dv[kGtSlabs];
+    uint32_t iv[kGtSlabs], rv[kGtSlabs];
+#pragma unroll
+    for (uint32_t s = 0; s < kGtSlabs; s++) {
+      const uint32_t e = l + s * kWave;
+      dv[s] = e < c ? D[e] : 0.0;
+      iv[s] = e < c ? I[e] : 0u;
+      rv[s] = 0;
+    }
     for (uint32_t t = 0; t < c; t++) {
       const double dt = D[t];
       const uint32_t it = I[t];
-      r0 += less_gt(dt, it, d0, i0) ? 1u : 0u;
-      r1 += less_gt(dt, it, d1, i1) ? 1u : 0u;
+#pragma unroll
+      for (uint32_t s = 0; s < kGtSlabs; s++) rv[s] += less_gt(dt, it, dv[s], iv[s]) ? 1u : 0u;
     }
     wave_sync();
-    if (h0 && r0 < k) { D[r0] = d0; I[r0] = i0; }
-    if (h1 && r1 < k) { D[r1] = d1; I[r1] = i1; }
+#pragma unroll
+    for (uint32_t s = 0; s < kGtSlabs; s++)
+      if (l + s * kWave < c && rv[s] < k) { D[rv[s]] = dv[s]; I[rv[s]] = iv[s]; }
     wave_sync();
     if (l == 0) {
       const uint32_t nc = c < k ? c : k;
@@ -329,19 +340,19 @@ __global__ __launch_bounds__(256, 2) void k_gt_f64(GtArgs g) {
   tk.finish(g, q0, blockIdx.y);
 }
 
-template <typename T, int METRIC>
-__global__ __launch_bounds__(64 * kGtWavesI8) void k_gt_i8(GtArgs g) {
+template <typename T, int METRIC, uint32_t WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_gt_i8(GtArgs g) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr bool U8 = sizeof(T) == 1 && T(-1) > T(0);
   constexpr uint32_t kFlip = U8 ? 0x80808080u : 0u;
   double* tile_pn = (double*)smem;
   double* tile_ps = tile_pn + kGtTile;
-  unsigned char* Bs = smem + gt_tile_off(g.cap, kGtWavesI8);
+  unsigned char* Bs = smem + gt_tile_off(g.cap, WAVES);
   const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
   const uint32_t g4 = lane >> 4, c16 = lane & 15u;
   GtTopk tk(smem + 1024u + wave * gt_wave_bytes(g.cap), g.cap, g.k, kGtTile);
   tk.init();
-  const uint64_t q0 = (uint64_t)blockIdx.x * (kGtWavesI8 * kGtRows) + wave * kGtRows;
+  const uint64_t q0 = (uint64_t)blockIdx.x * (WAVES * kGtRows) + wave * kGtRows;
   const uint32_t Q = g.Q, Q4 = 4u * Q, nsteps = Q / 16u, cpr = Q4 / 16u;
   const double P = (double)Q4;
 
@@ -485,27 +496,29 @@ __global__ __launch_bounds__(256) void k_gt_merge(const double* pd, const uint32
 
 struct GtPlan {
   bool mfma;
-  uint32_t Q, bstride, cap, S, lds;
+  uint32_t Q, bstride, cap, S, lds, waves;
   uint64_t per_slice;
 };
 
 static GtPlan gt_plan(int dtype, uint64_t n, uint64_t nq, uint32_t d, uint32_t k) {
   GtPlan p{};
-  p.cap = kGtTile + (k > 16 ? 32u : 16u);
+  p.cap = kGtTile + (k + 15u) / 16u * 16u;  // the k kept entries plus one tile's offers
   const uint32_t qd = (d + 3) / 4;
   if (dtype == F32) {
     p.mfma = qd <= kGtMaxQf;
     p.Q = qd;
     p.bstride = 4 * qd + 1;
+    p.waves = kGtWavesF;
     p.lds = gt_tile_off(p.cap, kGtWavesF) + kGtTile * p.bstride * 4;
   } else {
     p.Q = (qd + 15) / 16 * 16;
     p.mfma = p.Q / 16 <= kGtMaxSteps;
     p.bstride = 4 * p.Q + 16;
-    p.lds = gt_tile_off(p.cap, kGtWavesI8) + kGtTile * p.bstride;
+    p.waves = k > 32 ? kGtWavesI8Big : kGtWavesI8;
+    p.lds = gt_tile_off(p.cap, p.waves) + kGtTile * p.bstride;
   }
   if (p.lds > 160u * 1024u) p.mfma = false;  // beyond a CU's LDS: the per-query kernel
-  const uint32_t qb = (dtype == F32 ? kGtWavesF : kGtWavesI8) * kGtRows;  // queries per block
+  const uint32_t qb = p.waves * kGtRows;  // queries per block
   const uint64_t qtiles = (nq + qb - 1) / qb;
   const uint64_t tiles = (n + kGtTile - 1) / kGtTile;
   uint64_t S = (2048 + qtiles - 1) / qtiles;
@@ -564,14 +577,21 @@ static hipError_t gt_t(const void* base, uint64_t n, const void* queries, uint64
                      (double*)g.pn, (double*)g.psum);
   hipLaunchKernelGGL((k_gt_norms<T>), dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, s, (const T*)queries, nq, d,
                      ad, (double*)g.qn, (double*)g.qsum);
-  const uint32_t waves = sizeof(T) == 4 ? kGtWavesF : kGtWavesI8;
+  const uint32_t waves = p.waves;
   const dim3 grid((uint32_t)((nq + waves * kGtRows - 1) / (waves * kGtRows)), p.S);
   if constexpr (sizeof(T) == 4) {
     (void)hipFuncSetAttribute((const void*)&k_gt_f64<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
     hipLaunchKernelGGL((k_gt_f64<M>), grid, dim3(64 * kGtWavesF), p.lds, s, g);
   } else {
-    (void)hipFuncSetAttribute((const void*)&k_gt_i8<T, M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
-    hipLaunchKernelGGL((k_gt_i8<T, M>), grid, dim3(64 * kGtWavesI8), p.lds, s, g);
+    if (waves == kGtWavesI8) {
+      (void)hipFuncSetAttribute((const void*)&k_gt_i8<T, M, kGtWavesI8>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)p.lds);
+      hipLaunchKernelGGL((k_gt_i8<T, M, kGtWavesI8>), grid, dim3(64 * kGtWavesI8), p.lds, s, g);
+    } else {
+      (void)hipFuncSetAttribute((const void*)&k_gt_i8<T, M, kGtWavesI8Big>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)p.lds);
+      hipLaunchKernelGGL((k_gt_i8<T, M, kGtWavesI8Big>), grid, dim3(64 * kGtWavesI8Big), p.lds, s, g);
+    }
   }
   const uint32_t E = p.S * k;
   const uint32_t wpb = 4, mlds = wpb * ((E * 12u + 15u) & ~15u);

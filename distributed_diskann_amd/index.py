@@ -70,6 +70,21 @@ class Index:
             raise ValueError(f"graph rows must be {(self.n, self.R + 1)}")
         self._chk(lib().ddg_graph_set(self.h, _c(rows)))
 
+    def get_rows(self, ids) -> np.ndarray:
+        """Graph rows of `ids` ([degree, nbrs...] each)."""
+        ids = np.ascontiguousarray(np.asarray(ids, _U32))
+        out = np.empty((len(ids), self.R + 1), _U32)
+        self._chk(lib().ddg_graph_get_rows(self.h, _c(ids), len(ids), _c(out)))
+        return out
+
+    def set_rows(self, ids, rows: np.ndarray):
+        """Replaces the graph rows of `ids` (G[i].update_neighbors, batched)."""
+        ids = np.ascontiguousarray(np.asarray(ids, _U32))
+        rows = np.ascontiguousarray(rows, dtype=_U32)
+        if rows.shape != (len(ids), self.R + 1):
+            raise ValueError(f"rows must be {(len(ids), self.R + 1)}")
+        self._chk(lib().ddg_graph_set_rows(self.h, _c(ids), len(ids), _c(rows)))
+
     # ------------------------------------------------------------- build
     def build(self, R=None, L=64, alpha=1.2, two_pass=False, start_point=0, base=2.0, max_fraction=1.0,
               max_build=0, final_sort=True, bsp=False, visited_counts=False):
@@ -123,6 +138,38 @@ class Index:
         self._chk(lib().ddg_search(self.h, _c(Q), nq, _c(qid), _c(st), len(st), C.byref(qp), _c(pqb), _c(ids),
                                    _c(dists), _c(cmps), _c(vis)))
         return ids, dists, cmps, vis
+
+    def beam_search(self, Q: np.ndarray, k=0, beam=50, cut=1.35, limit=0, degree_limit=0, starts=(0,),
+                    query_ids=None, vcap=None):
+        """beam_search (beamSearch.h:41-184) of every row of Q: per query
+        ((frontier ids, frontier dists), (visited ids, visited dists),
+        dist_cmps), both lists sorted by (distance, id)."""
+        Q = np.ascontiguousarray(Q)
+        if Q.dtype != self.rows.dtype or Q.shape[1] != self.ad:
+            raise ValueError("queries must match the base dtype and aligned dimension")
+        nq = Q.shape[0]
+        st = np.ascontiguousarray(np.asarray(starts, _U32))
+        qid = None if query_ids is None else np.ascontiguousarray(np.asarray(query_ids, _U32))
+        fcap = max(beam, len(st))
+        vcap = 2 * beam + 64 if vcap is None else vcap
+        qp = QueryParamsC(k, beam, cut, limit, degree_limit, TRAV["native"], 0)
+        while True:
+            fi = np.zeros((max(nq, 1), fcap), _U32)
+            fd = np.zeros((max(nq, 1), fcap), np.float32)
+            fl = np.zeros(max(nq, 1), _U32)
+            vi = np.zeros((max(nq, 1), vcap), _U32)
+            vd = np.zeros((max(nq, 1), vcap), np.float32)
+            vl = np.zeros(max(nq, 1), _U32)
+            cm = np.zeros(max(nq, 1), np.uint64)
+            st_ = lib().ddg_beam_search(self.h, _c(Q), nq, _c(qid), _c(st), len(st), C.byref(qp), fcap, _c(fi),
+                                        _c(fd), _c(fl), vcap, _c(vi), _c(vd), _c(vl), _c(cm))
+            if st_ != 0 and nq and int(vl[:nq].max()) > vcap:
+                vcap = int(vl[:nq].max())  # the visited lists were longer: retry with room for them
+                continue
+            self._chk(st_)
+            break
+        return [((fi[i, :fl[i]].copy(), fd[i, :fl[i]].copy()), (vi[i, :vl[i]].copy(), vd[i, :vl[i]].copy()),
+                 int(cm[i])) for i in range(nq)]
 
     def search_device(self, Q, k, beam, out_ids, out_dists=None, cmps=None, visited=None, cut=1.35, limit=0,
                       degree_limit=0, starts=None, per_query_beam=None, beam_max=None, query_ids=None,
@@ -380,18 +427,32 @@ def searchAll(Query_Points: PointRange, G: Graph, Base_Points: PointRange, Query
     return ids
 
 
+def beam_search(p, G: Graph, Points: PointRange, starting_point, QP: QueryParams, p_id=None):
+    """beamSearch.h:41-184: ((frontier, visited), dist_cmps) for one point p
+    (an aligned row of Points' dtype); p_id is p.id() for the self-skip
+    (None: no point of Points is skipped).  Lists are (ids, distances)."""
+    idx = _device_index(Points, G)
+    idx.set_graph(G.rows)
+    starts = [starting_point] if np.isscalar(starting_point) else list(starting_point)
+    row = np.ascontiguousarray(np.asarray(p, Points.values.dtype).reshape(1, -1))
+    qid = None if p_id is None else [p_id]
+    (fr, vis, cmps), = idx.beam_search(row, QP.k, QP.beamSize, QP.cut, QP.limit, QP.degree_limit, starts,
+                                       query_ids=qid if qid is not None else [0xFFFFFFFF])
+    return (fr, vis), cmps
+
+
 def recall_with_ties(found: np.ndarray, gt_ids: np.ndarray, gt_dists: np.ndarray, k: int) -> float:
     """check_nn_recall.h:86-109: the first k ground-truth ids plus every later
     one at the k-th distance count as correct answers."""
     n = found.shape[0]
-    correct = 0
-    for i in range(n):
-        res = list(gt_ids[i, :k])
-        last = gt_dists[i, k - 1]
-        res += [int(g) for g, dd in zip(gt_ids[i, k:], gt_dists[i, k:]) if dd == last]
-        rep = set(int(x) for x in found[i, :k])
-        correct += sum(1 for r in res if int(r) in rep)
-    return correct / float(k * n)
+    gt_ids = np.asarray(gt_ids)
+    gt_dists = np.asarray(gt_dists)
+    # ground-truth entry l counts when l < k or its distance equals the k-th
+    counted = np.ones(gt_ids.shape, bool)
+    counted[:, k:] = gt_dists[:, k:] == gt_dists[:, k - 1:k]
+    rep = np.asarray(found)[:, :k].astype(np.int64)
+    hit = (gt_ids.astype(np.int64)[:, :, None] == rep[:, None, :]).any(axis=2)
+    return int((hit & counted).sum()) / float(k * n)
 
 
 @dataclass

@@ -63,6 +63,14 @@ CASES = [
     ("int8", "mips", 7_777, 33, 48, 32),
     ("uint8", "l2", 999, 200, 512, 5),
     ("int8", "l2", 1_500, 10, 600, 10),        # beyond the int8 tile budget: per-query kernel
+    # k = 100, the width of the reference's ground-truth files: longer
+    # candidate lists (three entries per lane in a compaction), four-wave
+    # int8 blocks
+    ("float32", "mips", 30_000, 150, 200, 100),
+    ("float32", "l2", 4_000, 70, 20, 100),
+    ("uint8", "l2", 40_000, 150, 128, 100),
+    ("int8", "l2", 20_000, 90, 100, 100),
+    ("float32", "l2", 1_000, 9, 300, 100),     # per-query kernel
 ]
 
 
@@ -90,6 +98,7 @@ def test_ground_truth_duplicates_and_ties(oracle):
     X = align_rows(base[rng.integers(0, 50, 4000)])  # many exact duplicates: ties broken by id
     Q = align_rows(base[rng.integers(0, 50, 77)])
     idx = Index(X, 16, "l2", 8)
-    ids, dd = idx.ground_truth(Q, 32)
-    oi, od = oracle.ground_truth(oracle.Points(X, 16, "l2"), Q, 32)
-    assert np.array_equal(ids, oi) and np.array_equal(dd, od)
+    for k in (32, 100):
+        ids, dd = idx.ground_truth(Q, k)
+        oi, od = oracle.ground_truth(oracle.Points(X, 16, "l2"), Q, k)
+        assert np.array_equal(ids, oi) and np.array_equal(dd, od)
