@@ -3,8 +3,8 @@
 The compute path is libddg.so (HIP kernels for gfx950, C-ABI in
 include/ddg.h); this package is its Python host side."""
 from ._lib import DdgError, lib
-from .index import (BuildParams, Graph, Index, PointRange, QueryParams, checkRecall, knn_index, nn_result,
-                    recall_with_ties, searchAll, stats)
+from .index import (BuildParams, Graph, Index, PointRange, QueryParams, beam_search, checkRecall, knn_index,
+                    nn_result, recall_with_ties, searchAll, stats)
 
-__all__ = ["DdgError", "lib", "BuildParams", "Graph", "Index", "PointRange", "QueryParams", "checkRecall",
+__all__ = ["DdgError", "lib", "BuildParams", "Graph", "Index", "PointRange", "QueryParams", "beam_search", "checkRecall",
            "knn_index", "nn_result", "recall_with_ties", "searchAll", "stats"]

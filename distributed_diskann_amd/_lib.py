@@ -20,7 +20,8 @@ EXPORTS = [
     "ddg_create", "ddg_destroy", "ddg_last_error", "ddg_graph_set", "ddg_graph_get", "ddg_build",
     "ddg_batch_insert", "ddg_robust_prune", "ddg_search", "ddg_search_device", "ddg_insert_search",
     "ddg_ground_truth", "ddg_sort_rows", "ddg_size", "ddg_max_degree", "ddg_dims", "ddg_aligned_dims",
-    "ddg_last_profile", "ddg_prepare_traversal", "ddg_distances", "ddg_autotune_search",
+    "ddg_last_profile", "ddg_prepare_traversal", "ddg_distances", "ddg_autotune_search", "ddg_beam_search",
+    "ddg_graph_set_rows", "ddg_graph_get_rows",
 ]
 
 
@@ -85,6 +86,9 @@ def lib():
         "ddg_prepare_traversal": ([vp, i32, P(dbl)], i32),
         "ddg_distances": ([vp, i32, vp, u64, vp, u32, vp], i32),
         "ddg_autotune_search": ([vp, vp, u64, P(QueryParamsC), vp, vp], i32),
+        "ddg_beam_search": ([vp, vp, u64, vp, vp, u32, P(QueryParamsC), u32, vp, vp, vp, u32, vp, vp, vp, vp], i32),
+        "ddg_graph_set_rows": ([vp, vp, u64, vp], i32),
+        "ddg_graph_get_rows": ([vp, vp, u64, vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

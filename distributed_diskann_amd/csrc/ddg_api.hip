@@ -31,8 +31,10 @@ struct Scratch {
 enum BufId {
   B_QUERIES, B_STARTS, B_QBEAM, B_QIDS, B_OUT_IDS, B_OUT_D, B_CMPS, B_VISN, B_VIS_IDS, B_VIS_D, B_FLAGS,
   B_INSERTS, B_NEW_IDS, B_NEW_LEN, B_OFFS, B_KEYS, B_VALS, B_KEYS2, B_VALS2, B_TMP, B_GFLAGS, B_GOFFS, B_GKEYS,
-  B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D, B_GTAB, B_BIGOFF, B_RPTS
+  B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D, B_GTAB, B_BIGOFF, B_RPTS,
+  B_FR_IDS, B_FR_D, B_FR_LEN, B_ROW_IDS, B_ROWS, B_COUNT
 };
+static_assert(B_COUNT <= 40, "scratch slots");
 
 // Device words of B_FLAGS (u32 index): [0] search work counter, [1] visited
 // list overflow, [4] error status (bit 0 degree overflow, bit 1 prune scratch
@@ -1159,30 +1161,25 @@ ddg_status ddg_robust_prune(ddg_index* idx, uint64_t np, const uint32_t* pts, co
   return DDG_OK;
 }
 
-// Search over device-resident queries/outputs (all pointers device memory).
-ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
-                             const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
-                             const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids, float* out_dists,
-                             uint64_t* dist_cmps, uint32_t* visited) {
-  // every host-side scalar is checked before anything is launched: the
-  // kernel trusts nstarts (<= 64 seeds one lane each), the frontier capacity
-  // (beam_max) and cut >= 1 (the exactness of the visited flag)
-  if (!idx || !qp) return fail(idx, DDG_EINVAL, "NULL argument");
-  if (nq && (!queries || !out_ids || !starts)) return fail(idx, DDG_EINVAL, "NULL argument");
-  if (nstarts == 0 || nstarts > 64) return fail(idx, DDG_EINVAL, "nstarts must be in [1, 64]");
-  if (qp->trav != DDG_TRAV_NATIVE && qp->trav != DDG_TRAV_BF16 && qp->trav != DDG_TRAV_FP8E4M3)
-    return fail(idx, DDG_EINVAL, "unknown traversal mode");
-  if (qp->trav != DDG_TRAV_NATIVE && idx->dtype != DDG_F32)
-    return fail(idx, DDG_EINVAL, "reduced-precision traversal needs a float32 index");
-  if (qp->trav != DDG_TRAV_NATIVE && qp->rerank != 0 && qp->rerank < qp->k)
-    return fail(idx, DDG_EINVAL, "rerank must be 0 (all) or at least k");
-  if (beam_max == 0 || qp->k > beam_max)
-    return fail(idx, DDG_EINVAL, "Error: beam search parameter Q smaller than k");  // beamSearch.h:327-331
-  if (!per_query_beam && (qp->beam == 0 || qp->beam > beam_max || qp->k > qp->beam))
-    return fail(idx, DDG_EINVAL, "beam must be in [k, beam_max]");
-  if (qp->k > 0 && qp->cut < 1.0 && idx->metric == DDG_L2)
-    return fail(idx, DDG_EINVAL, "cut below 1.0 is not supported");
-  if (nq >= (1ull << 32)) return fail(idx, DDG_EINVAL, "nq must be below 2^32");
+// Outputs of a search launch beyond searchAll's: beam_search's whole final
+// frontier and its visited list (visit order; sorted on the host).
+struct SearchExtra {
+  uint32_t* fr_ids = nullptr;
+  float* fr_dists = nullptr;
+  uint32_t* fr_len = nullptr;
+  uint32_t fr_stride = 0;
+  uint32_t* vis_ids = nullptr;
+  float* vis_dists = nullptr;
+  uint32_t vcap = 0;
+  bool overflow = false;  // out: some visited list outgrew vcap
+};
+
+// One search launch over device-resident queries and outputs; the caller
+// has checked every argument.
+static ddg_status search_launch(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
+                                const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
+                                const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids,
+                                float* out_dists, uint64_t* dist_cmps, uint32_t* visited, SearchExtra* ex) {
   SearchArgs a{};
   Launch l{};
   const int trav = (int)qp->trav;
@@ -1219,6 +1216,15 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   a.build_mode = 0;
   a.cut_trim = idx->metric == DDG_L2 ? 1 : 0;
   a.rerank = qp->rerank;
+  if (ex) {
+    a.fr_ids = ex->fr_ids;
+    a.fr_dists = ex->fr_dists;
+    a.fr_len = ex->fr_len;
+    a.fr_stride = ex->fr_stride;
+    a.vis_ids = ex->vis_ids;
+    a.vis_dists = ex->vis_dists;
+    a.vcap = ex->vcap;
+  }
   l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (nq + l.waves_per_block - 1) / l.waves_per_block);
   if (l.blocks == 0) return DDG_OK;
   ENSURE_GTABLE(a, l);
@@ -1228,6 +1234,165 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   uint32_t ov = 0;
   HIPCHK(hipMemcpy(&ov, flags + 1, 4, hipMemcpyDeviceToHost));
   if (ov & 2u) return fail(idx, DDG_ERANGE, "start point out of range");
+  if (ex) ex->overflow = (ov & 1u) != 0;
+  return DDG_OK;
+}
+
+// Search over device-resident queries/outputs (all pointers device memory).
+ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
+                             const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
+                             const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids, float* out_dists,
+                             uint64_t* dist_cmps, uint32_t* visited) {
+  // every host-side scalar is checked before anything is launched: the
+  // kernel trusts nstarts (<= 64 seeds one lane each), the frontier capacity
+  // (beam_max) and cut >= 1 (the exactness of the visited flag)
+  if (!idx || !qp) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nq && (!queries || !out_ids || !starts)) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nstarts == 0 || nstarts > 64) return fail(idx, DDG_EINVAL, "nstarts must be in [1, 64]");
+  if (qp->trav != DDG_TRAV_NATIVE && qp->trav != DDG_TRAV_BF16 && qp->trav != DDG_TRAV_FP8E4M3)
+    return fail(idx, DDG_EINVAL, "unknown traversal mode");
+  if (qp->trav != DDG_TRAV_NATIVE && idx->dtype != DDG_F32)
+    return fail(idx, DDG_EINVAL, "reduced-precision traversal needs a float32 index");
+  if (qp->trav != DDG_TRAV_NATIVE && qp->rerank != 0 && qp->rerank < qp->k)
+    return fail(idx, DDG_EINVAL, "rerank must be 0 (all) or at least k");
+  if (beam_max == 0 || qp->k > beam_max)
+    return fail(idx, DDG_EINVAL, "Error: beam search parameter Q smaller than k");  // beamSearch.h:327-331
+  if (!per_query_beam && (qp->beam == 0 || qp->beam > beam_max || qp->k > qp->beam))
+    return fail(idx, DDG_EINVAL, "beam must be in [k, beam_max]");
+  if (qp->k > 0 && qp->cut < 1.0 && idx->metric == DDG_L2)
+    return fail(idx, DDG_EINVAL, "cut below 1.0 is not supported");
+  if (nq >= (1ull << 32)) return fail(idx, DDG_EINVAL, "nq must be below 2^32");
+  return search_launch(idx, queries, nq, query_ids, starts, nstarts, qp, per_query_beam, beam_max, out_ids,
+                       out_dists, dist_cmps, visited, nullptr);
+}
+
+ddg_status ddg_beam_search(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
+                           const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp, uint32_t fcap,
+                           uint32_t* frontier_ids, float* frontier_dists, uint32_t* frontier_len, uint32_t vcap,
+                           uint32_t* visited_ids, float* visited_dists, uint32_t* visited_len, uint64_t* dist_cmps) {
+  if (!idx || !qp || !starts) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nq && (!queries || !frontier_ids || !frontier_dists || !frontier_len || !visited_len))
+    return fail(idx, DDG_EINVAL, "NULL argument");
+  if (vcap && (!visited_ids || !visited_dists)) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nstarts == 0 || nstarts > 64) return fail(idx, DDG_EINVAL, "nstarts must be in [1, 64]");
+  if (qp->trav != DDG_TRAV_NATIVE) return fail(idx, DDG_EINVAL, "beam_search traverses the native rows only");
+  if (qp->beam == 0) return fail(idx, DDG_EINVAL, "beam must be positive");
+  if (fcap < std::max(qp->beam, nstarts)) return fail(idx, DDG_EINVAL, "fcap must be at least max(beam, nstarts)");
+  if (qp->k > 0 && qp->cut < 1.0 && idx->metric == DDG_L2)
+    return fail(idx, DDG_EINVAL, "cut below 1.0 is not supported");
+  if (nq >= (1ull << 32)) return fail(idx, DDG_EINVAL, "nq must be below 2^32");
+  for (uint32_t i = 0; i < nstarts; i++)
+    if (starts[i] >= idx->n) return fail(idx, DDG_ERANGE, "start point out of range");
+  if (nq == 0) return DDG_OK;
+  for (double& p : idx->prof) p = 0;
+  const size_t row_bytes = (size_t)idx->ad * idx->es;
+  const uint32_t fs = std::max(qp->beam, nstarts);
+  ENSURE(B_QUERIES, nq * row_bytes);
+  ENSURE(B_STARTS, 256);
+  ENSURE(B_CMPS, nq * 8);
+  ENSURE(B_VISN, nq * 4);
+  ENSURE(B_FR_IDS, nq * fs * 4);
+  ENSURE(B_FR_D, nq * fs * 4);
+  ENSURE(B_FR_LEN, nq * 4);
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_QUERIES), queries, nq * row_bytes, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(launch_pad_rows(buf<void>(idx, B_QUERIES), nq, idx->d * idx->es, (uint32_t)row_bytes, idx->stream));
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_STARTS), starts, nstarts * 4, hipMemcpyHostToDevice, idx->stream));
+  uint32_t* dqid = nullptr;
+  if (query_ids) {
+    ENSURE(B_QIDS, nq * 4);
+    dqid = buf<uint32_t>(idx, B_QIDS);
+    HIPCHK(hipMemcpyAsync(dqid, query_ids, nq * 4, hipMemcpyHostToDevice, idx->stream));
+  }
+  // the visited lists are recorded into device slots of vdev entries; a list
+  // that outgrows them reruns the batch with twice the room (the limit
+  // bounds every list), so visited_len always holds the true lengths
+  const uint64_t lim = qp->limit ? std::min<uint64_t>(qp->limit, idx->n) : idx->n;
+  uint32_t vdev = (uint32_t)std::max<uint64_t>(std::min<uint64_t>(std::max<uint32_t>(vcap, 2 * qp->beam + 64), lim), 1);
+  SearchExtra ex;
+  for (;;) {
+    ENSURE(B_VIS_IDS, nq * (size_t)vdev * 4);
+    ENSURE(B_VIS_D, nq * (size_t)vdev * 4);
+    ex.fr_ids = buf<uint32_t>(idx, B_FR_IDS);
+    ex.fr_dists = buf<float>(idx, B_FR_D);
+    ex.fr_len = buf<uint32_t>(idx, B_FR_LEN);
+    ex.fr_stride = fs;
+    ex.vis_ids = buf<uint32_t>(idx, B_VIS_IDS);
+    ex.vis_dists = buf<float>(idx, B_VIS_D);
+    ex.vcap = vdev;
+    ddg_status st = search_launch(idx, buf<void>(idx, B_QUERIES), nq, dqid, buf<uint32_t>(idx, B_STARTS), nstarts,
+                                  qp, nullptr, qp->beam, nullptr, nullptr, buf<uint64_t>(idx, B_CMPS),
+                                  buf<uint32_t>(idx, B_VISN), &ex);
+    if (st != DDG_OK) return st;
+    if (!ex.overflow) break;
+    if (vdev >= lim) return fail(idx, DDG_EDEVICE, "visited list longer than the visit limit");
+    vdev = (uint32_t)std::min<uint64_t>(2ull * vdev, lim);
+  }
+  std::vector<uint32_t> fi((size_t)nq * fs), fl(nq), vi((size_t)nq * vdev), vl(nq);
+  std::vector<float> fd((size_t)nq * fs), vd((size_t)nq * vdev);
+  HIPCHK(hipMemcpy(fi.data(), buf<void>(idx, B_FR_IDS), fi.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(fd.data(), buf<void>(idx, B_FR_D), fd.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(fl.data(), buf<void>(idx, B_FR_LEN), nq * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vi.data(), buf<void>(idx, B_VIS_IDS), vi.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vd.data(), buf<void>(idx, B_VIS_D), vd.size() * 4, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(vl.data(), buf<void>(idx, B_VISN), nq * 4, hipMemcpyDeviceToHost));
+  if (dist_cmps) HIPCHK(hipMemcpy(dist_cmps, buf<void>(idx, B_CMPS), nq * 8, hipMemcpyDeviceToHost));
+  bool short_cap = false;
+  std::vector<std::pair<float, uint32_t>> tmp;
+  for (uint64_t q = 0; q < nq; q++) {
+    frontier_len[q] = fl[q];
+    std::memcpy(frontier_ids + q * fcap, fi.data() + q * fs, fl[q] * 4);
+    std::memcpy(frontier_dists + q * fcap, fd.data() + q * fs, fl[q] * 4);
+    visited_len[q] = vl[q];
+    if (vl[q] > vcap) { short_cap = true; continue; }
+    // beam_search keeps visited sorted by (distance, id) (beamSearch.h:115-117)
+    tmp.resize(vl[q]);
+    for (uint32_t j = 0; j < vl[q]; j++) tmp[j] = {vd[q * vdev + j], vi[q * vdev + j]};
+    std::sort(tmp.begin(), tmp.end(), [](const std::pair<float, uint32_t>& x, const std::pair<float, uint32_t>& y) {
+      return x.first < y.first || (x.first == y.first && x.second < y.second);
+    });
+    for (uint32_t j = 0; j < vl[q]; j++) {
+      visited_ids[q * vcap + j] = tmp[j].second;
+      visited_dists[q * vcap + j] = tmp[j].first;
+    }
+  }
+  if (short_cap) return fail(idx, DDG_EINVAL, "a visited list is longer than vcap (visited_len holds the lengths)");
+  return DDG_OK;
+}
+
+ddg_status ddg_graph_set_rows(ddg_index* idx, const uint32_t* ids, uint64_t m, const uint32_t* rows) {
+  if (!idx || (m && (!ids || !rows))) return fail(idx, DDG_EINVAL, "NULL argument");
+  const uint32_t R1 = idx->R + 1;
+  for (uint64_t i = 0; i < m; i++) {
+    if (ids[i] >= idx->n) return fail(idx, DDG_ERANGE, "row id out of range");
+    const uint32_t* r = rows + i * (uint64_t)R1;
+    if (r[0] > idx->R) return fail(idx, DDG_EDEGREE, "graph row degree exceeds R");
+    for (uint32_t j = 0; j < r[0]; j++)
+      if (r[1 + j] >= idx->n) return fail(idx, DDG_ERANGE, "graph row holds an id >= n");
+  }
+  if (m == 0) return DDG_OK;
+  ENSURE(B_ROW_IDS, m * 4);
+  ENSURE(B_ROWS, m * (size_t)R1 * 4);
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_ROW_IDS), ids, m * 4, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_ROWS), rows, m * (size_t)R1 * 4, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(launch_copy_rows(idx->graph, R1, buf<uint32_t>(idx, B_ROW_IDS), m, buf<uint32_t>(idx, B_ROWS), 1,
+                          idx->stream));
+  HIPCHK(hipStreamSynchronize(idx->stream));
+  return DDG_OK;
+}
+
+ddg_status ddg_graph_get_rows(ddg_index* idx, const uint32_t* ids, uint64_t m, uint32_t* rows) {
+  if (!idx || (m && (!ids || !rows))) return fail(idx, DDG_EINVAL, "NULL argument");
+  for (uint64_t i = 0; i < m; i++)
+    if (ids[i] >= idx->n) return fail(idx, DDG_ERANGE, "row id out of range");
+  if (m == 0) return DDG_OK;
+  const uint32_t R1 = idx->R + 1;
+  ENSURE(B_ROW_IDS, m * 4);
+  ENSURE(B_ROWS, m * (size_t)R1 * 4);
+  HIPCHK(hipMemcpyAsync(buf<void>(idx, B_ROW_IDS), ids, m * 4, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(launch_copy_rows(idx->graph, R1, buf<uint32_t>(idx, B_ROW_IDS), m, buf<uint32_t>(idx, B_ROWS), 0,
+                          idx->stream));
+  HIPCHK(hipMemcpyAsync(rows, buf<void>(idx, B_ROWS), m * (size_t)R1 * 4, hipMemcpyDeviceToHost, idx->stream));
+  HIPCHK(hipStreamSynchronize(idx->stream));
   return DDG_OK;
 }
 
@@ -1589,7 +1754,7 @@ ddg_status ddg_internal_group_pairs(ddg_index* idx, const uint32_t* keys, const 
 ddg_status ddg_ground_truth(ddg_index* idx, const void* queries, uint64_t nq, uint32_t k, uint32_t* ids,
                             double* dists) {
   if (!idx || (nq && (!queries || !ids))) return fail(idx, DDG_EINVAL, "NULL argument");
-  if (k == 0 || k > 32 || k > idx->n) return fail(idx, DDG_EINVAL, "k must be in [1, min(32, n)]");
+  if (k == 0 || k > 100 || k > idx->n) return fail(idx, DDG_EINVAL, "k must be in [1, min(100, n)]");
   if (nq == 0) return DDG_OK;
   for (double& p : idx->prof) p = 0;
   const size_t row_bytes = (size_t)idx->ad * idx->es;

@@ -29,6 +29,26 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Graph rows ids[r] <-> packed row r of `rows` (R1 words each), one thread per
+// word: a batch of G[i].update_neighbors / G[i] reads (graph.h:72-81).
+__global__ void k_copy_rows(uint32_t* graph, uint32_t R1, const uint32_t* ids, uint64_t m, uint32_t* rows,
+                            int to_graph) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m * R1) return;
+  const uint64_t r = i / R1, c = i - r * R1;
+  uint32_t* g = graph + (uint64_t)ids[r] * R1 + c;
+  if (to_graph) *g = rows[i];
+  else rows[i] = *g;
+}
+
+hipError_t launch_copy_rows(uint32_t* graph, uint32_t R1, const uint32_t* ids, uint64_t m, uint32_t* rows,
+                            int to_graph, hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_rows, dim3(blocks_for(m * R1, 256)), dim3(256), 0, s, graph, R1, ids, m, rows,
+                     to_graph);
+  return hipGetLastError();
+}
+
 // zero bytes [d_bytes, row_bytes) of every row
 __global__ void k_pad_rows(unsigned char* rows, uint64_t n, uint32_t d_bytes, uint32_t row_bytes) {
   const uint32_t pad = row_bytes - d_bytes;

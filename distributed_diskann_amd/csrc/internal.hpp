@@ -21,8 +21,12 @@ struct SearchArgs {
   float* out_dists;              // nullable
   uint64_t* dist_cmps;           // nullable, nq
   uint32_t* visited_count;       // nullable, nq
-  uint32_t* vis_ids;             // build mode: nq x vcap, visit order
+  uint32_t* vis_ids;             // nullable (set in build mode): nq x vcap visited (id, dist), visit order
   float* vis_dists;
+  uint32_t* fr_ids;              // nullable: the whole final frontier, nq x fr_stride (native traversal)
+  float* fr_dists;
+  uint32_t* fr_len;
+  uint32_t fr_stride;            // >= max(beam, nstarts): a search that never expands keeps all its starts
   uint32_t* overflow;            // build mode: set to 1 when a list exceeds vcap
   uint32_t* work;                // work-queue counter, zero at launch
   uint64_t n, nq, limit;
@@ -133,6 +137,9 @@ hipError_t launch_group_starts(const uint32_t* keys, uint64_t n, uint32_t* flags
 hipError_t launch_sort_rows(int dtype, int metric, const void* base, uint32_t* graph, uint64_t n, uint32_t R1,
                             uint32_t d, uint32_t ad, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+// graph rows ids[0..m) to (to_graph = 0) or from (to_graph = 1) the packed m x R1 array rows
+hipError_t launch_copy_rows(uint32_t* graph, uint32_t R1, const uint32_t* ids, uint64_t m, uint32_t* rows,
+                            int to_graph, hipStream_t s);
 hipError_t launch_pad_rows(void* rows, uint64_t n, uint32_t d_bytes, uint32_t row_bytes, hipStream_t s);
 
 // dense.hip

@@ -239,7 +239,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 
       wave_sync();
       if (lane == 0) {
         ((uint32_t*)FR)[2 * pos + 1] = cid | kFlag;
-        if (a.build_mode) {
+        if (a.vis_ids) {
           if (nvis < a.vcap) {
             a.vis_ids[(uint64_t)q * a.vcap + nvis] = cid;
             a.vis_dists[(uint64_t)q * a.vcap + nvis] = cdist;
@@ -644,11 +644,19 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 
       }
     } else if (!a.build_mode) {
       const uint2* FR = cur ? F1 : F0;
-      for (uint32_t j = lane; j < a.k; j += kWave) {
+      for (uint32_t j = lane; a.out_ids && j < a.k; j += kWave) {
         const bool ok = j < fsz;
         const uint2 e = ok ? FR[j] : make_uint2(0u, 0u);
         a.out_ids[(uint64_t)q * a.k + j] = ok ? (e.y & kIdMask) : 0xFFFFFFFFu;
         if (a.out_dists) a.out_dists[(uint64_t)q * a.k + j] = ok ? __uint_as_float(e.x) : __builtin_inff();
+      }
+      if (a.fr_ids) {  // beam_search's frontier (beamSearch.h:183)
+        for (uint32_t j = lane; j < fsz; j += kWave) {
+          const uint2 e = FR[j];
+          a.fr_ids[(uint64_t)q * a.fr_stride + j] = e.y & kIdMask;
+          a.fr_dists[(uint64_t)q * a.fr_stride + j] = __uint_as_float(e.x);
+        }
+        if (lane == 0) a.fr_len[q] = fsz;
       }
     }
     if (lane == 0) {

@@ -140,6 +140,30 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
                              const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids, float* out_dists,
                              uint64_t* dist_cmps, uint32_t* visited);
 
+/* beam_search (beamSearch.h:41-54 signature, 50-184 algorithm) for nq points
+ * in one launch, with everything the reference returns,
+ * ((frontier, visited), dist_cmps): query i's final frontier (frontier_len[i]
+ * <= max(beam, nstarts) (id, distance) pairs sorted by (distance, id)) at
+ * frontier_ids / frontier_dists + i*fcap, its visited list sorted by
+ * (distance, id) (beamSearch.h:115-117) at visited_ids / visited_dists +
+ * i*vcap, and dist_cmps[i] (nullable).  query_ids gives p.id() for the
+ * self-skip of beamSearch.h:128 (NULL = the query's row index); qp->k only
+ * drives the cut trim (k = 0 is the build mode of index.h:277); native
+ * traversal only.  fcap >= max(beam, nstarts).  visited_len[i] always holds
+ * the true length; a list longer than vcap fails the call with DDG_EINVAL
+ * (the caller can retry with a larger vcap). */
+ddg_status ddg_beam_search(ddg_index* idx, const void* queries /* nq*aligned_d */, uint64_t nq,
+                           const uint32_t* query_ids, const uint32_t* starts, uint32_t nstarts,
+                           const ddg_query_params* qp, uint32_t fcap, uint32_t* frontier_ids, float* frontier_dists,
+                           uint32_t* frontier_len, uint32_t vcap, uint32_t* visited_ids, float* visited_dists,
+                           uint32_t* visited_len, uint64_t* dist_cmps);
+
+/* Graph rows by id (graph.h:43-117: G[i] reads and G[i].update_neighbors,
+ * batched): rows is m packed rows of R+1 words [degree, nbrs...].  set
+ * checks every degree <= R and every id < n before writing. */
+ddg_status ddg_graph_set_rows(ddg_index* idx, const uint32_t* ids, uint64_t m, const uint32_t* rows);
+ddg_status ddg_graph_get_rows(ddg_index* idx, const uint32_t* ids, uint64_t m, uint32_t* rows);
+
 /* Build-mode traces: beam_search(Points[ids[i]], G, Points, start,
  * QueryParams(0, L, 0, n, R)) as batch_insert calls it (index.h:277-279).
  * Writes |visited| and the visited (id, distance) lists, vcap per point, in
@@ -148,9 +172,10 @@ ddg_status ddg_insert_search(ddg_index* idx, const uint32_t* ids, uint64_t m, ui
                              uint32_t vcap, uint32_t* vis_ids, float* vis_dists, uint32_t* vis_len,
                              uint64_t* dist_cmps);
 
-/* Brute-force top-k of nq queries over the index's vectors: FP64
- * accumulation for float32 (MFMA tiles), exact integer for uint8/int8, ties
- * by id.  dists may be NULL. */
+/* Brute-force top-k (k <= 100, the width of the reference's ground-truth
+ * files) of nq queries over the index's vectors: FP64 accumulation for
+ * float32 (MFMA tiles), exact integer for uint8/int8, ties by id.  dists may
+ * be NULL. */
 ddg_status ddg_ground_truth(ddg_index* idx, const void* queries /* nq*aligned_d */, uint64_t nq, uint32_t k,
                             uint32_t* ids, double* dists);
 

@@ -1,6 +1,7 @@
-// Test program for include/ddg/knn_index.hpp: the build and query calls of
-// src/diskann.cpp:41-81 and check_nn_recall.h:52-67, written against the
-// header-only adapter exactly as a caller of the reference would write them.
+// Test program for include/ddg/knn_index.hpp with its standalone types: the
+// build and query calls of src/diskann.cpp:41-81 and check_nn_recall.h:52-67,
+// written against the header-only adapter exactly as a caller of the
+// reference would write them.
 //
 //   adapter_demo <base.u8bin> <queries.u8bin> <out_graph> <out_ids> R L alpha k beam
 //
@@ -20,19 +21,21 @@ int main(int argc, char** argv) {
   }
   const long R = atol(argv[5]), L = atol(argv[6]), k = atol(argv[8]), beam = atol(argv[9]);
   const double alpha = atof(argv[7]);
+  using Point = ddg::Euclidian_Point<uint8_t>;
+  using PR = ddg::PointRange<uint8_t, Point>;
 
   ddg::BuildParams BP(R, L, alpha, false);
-  ddg::PointRange<uint8_t> Points(argv[1]);
+  PR Points(argv[1]);
   ddg::Graph G(BP.R, Points.size());
   ddg::stats<unsigned int> BuildStats(G.size());
-  ddg::knn_index<uint8_t> I(BP);
+  ddg::knn_index<Point, PR, unsigned int> I(BP);
   I.build_index(G, Points, BuildStats);
   G.save(argv[3]);
 
-  ddg::PointRange<uint8_t> Query(argv[2]);
+  PR Query(argv[2]);
   ddg::stats<unsigned int> QueryStats(Query.size());
   ddg::QueryParams QP(k, beam, 1.35, (long)G.size(), G.max_degree());
-  auto res = ddg::searchAll<uint8_t, ddg::Euclidian, unsigned int>(Query, G, Points, QueryStats, 0u, QP);
+  auto res = ddg::searchAll<Point, PR, unsigned int>(Query, G, Points, QueryStats, 0u, QP);
   FILE* f = std::fopen(argv[4], "wb");
   if (!f) return 3;
   for (auto& r : res) std::fwrite(r.data(), 4, r.size(), f);

@@ -55,3 +55,48 @@ def test_adapter_matches_python_mirror(tmp_path):
     Qa, _ = formats.read_vectors(qf, "uint8")
     want = idx.search(Qa, 10, 50, 1.35)[0]
     assert np.array_equal(ids, want)
+
+
+DROPIN = os.path.join(ROOT, "oracle", "_ref", "dropin_demo")
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("Ls", [20, 100])
+def test_dropin_on_reference_types(tmp_path, oracle, xs, Ls):
+    """The adapter on the reference's own Graph<unsigned> / PointRange<float,
+    Euclidian_Point<float>> (tests/cpp/dropin_demo.cpp, built from the
+    reference's headers): beam_search (per point and batched) and searchAll
+    equal the reference's own in the same process, bit for bit, also after a
+    graph row is changed on the host; robustPrune of the reference's
+    build-mode visited lists and build_index into a reference Graph equal the
+    oracle's (distance, id)-ordered results."""
+    if not os.path.exists(DROPIN):
+        pytest.skip("oracle/_ref/dropin_demo is built only where the reference tree is present")
+    print(f"[gpu] drop-in adapter on the reference's types, Ls={Ls}", flush=True)
+    z = xs["z"]
+    bf, qf, gf = tmp_path / "base", tmp_path / "queries", tmp_path / "graph"
+    z["base_file"].tofile(bf)
+    z["query_file"].tofile(qf)
+    z["graph_file"].tofile(gf)
+    out = str(tmp_path / "out")
+    r = subprocess.run([DROPIN, str(bf), str(qf), str(gf), out, str(Ls), "32", "50", "1.2"], capture_output=True,
+                       text=True, timeout=500)
+    print(r.stdout[-2000:], flush=True)
+    assert r.returncode == 0, (r.returncode, r.stdout[-3000:], r.stderr[-3000:])
+    P = oracle.Points(xs["X"], xs["d"])
+    G = oracle.Graph(xs["G"])
+    raw = np.fromfile(out + ".prune", np.uint32)
+    pos = 0
+    for p in range(200):
+        assert raw[pos] == p
+        ln = int(raw[pos + 1])
+        got = raw[pos + 2:pos + 2 + ln]
+        pos += 2 + ln
+        (_, _), (vi, vd), _ = oracle.beam_search(P, G, xs["X"][p], p, [0], 0, 50, 0.0, xs["X"].shape[0], 32)
+        want = oracle.robust_prune(P, G, p, vi, vd, 1.2, add=True, R=32, tie_mode=oracle.TIE_ID)
+        assert np.array_equal(got, want), p
+    from distributed_diskann_amd import formats
+    built = formats.read_graph(out + ".graph")
+    want = oracle.build(P, 32, 50, 1.2, tie_mode=oracle.TIE_ID)
+    assert np.array_equal(canon(built), canon(want.rows))
