@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,6 +71,8 @@ struct ddg_index {
   std::vector<Tuned> tuned;
   const Tuned* cfg_override = nullptr;  // set while the autotuner times a shape
   uint32_t num_cu = 256;
+  unsigned char* noalloc_scratch = nullptr;  // this GPU's copy of `scratch` (null: its address was not found)
+  std::string noalloc_err;
   Scratch s;
   std::string err;
   double prof[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -876,6 +879,37 @@ static ddg_status prepare_trav(ddg_index* idx, int trav, double* seconds) {
   return DDG_OK;
 }
 
+// ------------------------------------------------ allocation-free search
+// Scratch of ddg_search_device_noalloc: the 256-byte status block (work
+// counter, status word) and then the global visited-filter slices of beams
+// above 90.  One copy of the array per GPU, held until the process ends and
+// shared by every call on that GPU (calls that overlap on one GPU race).  In
+// host code `scratch` names a host-side stand-in, never the device memory:
+// only the address hipGetSymbolAddress returns, once per device with that
+// device current, is passed to kernels and fills.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+static bool g_scratch_done[kMaxDevices];
+
+// This device's copy of `scratch` (looked up on the device's first handle).
+static unsigned char* noalloc_scratch_of(int dev, hipError_t* err) {
+  if (dev < 0 || dev >= kMaxDevices) { *err = hipErrorInvalidDevice; return nullptr; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_done[dev]) {
+    void* p = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? (unsigned char*)p : nullptr;
+    g_scratch_done[dev] = true;
+  }
+  *err = g_scratch_err[dev];
+  return g_scratch_ptr[dev];
+}
+
 // ====================================================================== ABI
 extern "C" {
 
@@ -914,6 +948,11 @@ ddg_status ddg_create(ddg_dtype dtype, ddg_metric metric, uint64_t n, uint32_t d
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     idx->num_cu = (uint32_t)prop.multiProcessorCount;
+  {
+    hipError_t se = hipSuccess;
+    idx->noalloc_scratch = noalloc_scratch_of(dev, &se);
+    if (!idx->noalloc_scratch) idx->noalloc_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+  }
   ddg_status st = DDG_OK;
   auto chk = [&](hipError_t e, const char* what) {
     if (e != hipSuccess && st == DDG_OK) {
@@ -1176,19 +1215,33 @@ struct SearchExtra {
 
 // One search launch over device-resident queries and outputs; the caller
 // has checked every argument.
+// `fixed` (nullable): a caller-provided scratch of fixed_bytes that takes the
+// status words and the global filter slices instead of the handle's own
+// buffers; the launch then allocates nothing.
 static ddg_status search_launch(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
                                 const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
                                 const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids,
-                                float* out_dists, uint64_t* dist_cmps, uint32_t* visited, SearchExtra* ex) {
+                                float* out_dists, uint64_t* dist_cmps, uint32_t* visited, SearchExtra* ex,
+                                unsigned char* fixed = nullptr, size_t fixed_bytes = 0) {
   SearchArgs a{};
   Launch l{};
   const int trav = (int)qp->trav;
-  ddg_status pst = prepare_trav(idx, trav, nullptr);
-  if (pst != DDG_OK) return pst;
+  if (fixed) {
+    if (trav != DDG_TRAV_NATIVE && !idx->trav_rows[trav])
+      return fail(idx, DDG_EINVAL, "the reduced traversal copy does not exist: call ddg_prepare_traversal first");
+  } else {
+    ddg_status pst = prepare_trav(idx, trav, nullptr);
+    if (pst != DDG_OK) return pst;
+  }
   if (!search_layout(idx, beam_max, nstarts, trav, a, l))
     return fail(idx, DDG_EINVAL, "beam width too large for the LDS frontier");
-  ENSURE(B_FLAGS, kFlagBytes);
-  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+  uint32_t* flags = nullptr;
+  if (fixed) {
+    flags = (uint32_t*)fixed;
+  } else {
+    ENSURE(B_FLAGS, kFlagBytes);
+    flags = buf<uint32_t>(idx, B_FLAGS);
+  }
   HIPCHK(launch_fill_u32(flags, 0u, 2, idx->stream));
   a.base = idx->rows;
   a.graph = idx->graph;
@@ -1227,7 +1280,21 @@ static ddg_status search_launch(ddg_index* idx, const void* queries, uint64_t nq
   }
   l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (nq + l.waves_per_block - 1) / l.waves_per_block);
   if (l.blocks == 0) return DDG_OK;
-  ENSURE_GTABLE(a, l);
+  if (fixed) {
+    if (a.gtable_words) {
+      // as many resident blocks as whole filter slices fit after the status
+      // words; the launch shape does not change results
+      const size_t slice = (size_t)a.gtable_words * 4, per_block = slice * l.waves_per_block;
+      const size_t fit = (fixed_bytes - kFlagBytes) / per_block;
+      if (fit == 0)
+        return fail(idx, DDG_EINVAL, "beam too large for the allocation-free search: one block's visited filters "
+                                     "exceed its scratch");
+      l.blocks = (uint32_t)std::min<size_t>(l.blocks, fit);
+      a.gtable = (uint32_t*)(fixed + kFlagBytes);
+    }
+  } else {
+    ENSURE_GTABLE(a, l);
+  }
   Timer t(idx, &idx->prof[0]);
   HIPCHK(launch_search(idx->dtype, idx->metric, trav, a, l, idx->stream));
   t.stop();
@@ -1264,6 +1331,31 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
   if (nq >= (1ull << 32)) return fail(idx, DDG_EINVAL, "nq must be below 2^32");
   return search_launch(idx, queries, nq, query_ids, starts, nstarts, qp, per_query_beam, beam_max, out_ids,
                        out_dists, dist_cmps, visited, nullptr);
+}
+
+ddg_status ddg_search_device_noalloc(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
+                                     const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
+                                     const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids,
+                                     float* out_dists, uint64_t* dist_cmps, uint32_t* visited) {
+  if (!idx || !qp) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (!idx->noalloc_scratch) return fail(idx, DDG_EDEVICE, idx->noalloc_err);
+  if (nq && (!queries || !out_ids || !starts)) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nstarts == 0 || nstarts > 64) return fail(idx, DDG_EINVAL, "nstarts must be in [1, 64]");
+  if (qp->trav != DDG_TRAV_NATIVE && qp->trav != DDG_TRAV_BF16 && qp->trav != DDG_TRAV_FP8E4M3)
+    return fail(idx, DDG_EINVAL, "unknown traversal mode");
+  if (qp->trav != DDG_TRAV_NATIVE && idx->dtype != DDG_F32)
+    return fail(idx, DDG_EINVAL, "reduced-precision traversal needs a float32 index");
+  if (qp->trav != DDG_TRAV_NATIVE && qp->rerank != 0 && qp->rerank < qp->k)
+    return fail(idx, DDG_EINVAL, "rerank must be 0 (all) or at least k");
+  if (beam_max == 0 || qp->k > beam_max)
+    return fail(idx, DDG_EINVAL, "Error: beam search parameter Q smaller than k");  // beamSearch.h:327-331
+  if (!per_query_beam && (qp->beam == 0 || qp->beam > beam_max || qp->k > qp->beam))
+    return fail(idx, DDG_EINVAL, "beam must be in [k, beam_max]");
+  if (qp->k > 0 && qp->cut < 1.0 && idx->metric == DDG_L2)
+    return fail(idx, DDG_EINVAL, "cut below 1.0 is not supported");
+  if (nq >= (1ull << 32)) return fail(idx, DDG_EINVAL, "nq must be below 2^32");
+  return search_launch(idx, queries, nq, query_ids, starts, nstarts, qp, per_query_beam, beam_max, out_ids,
+                       out_dists, dist_cmps, visited, nullptr, idx->noalloc_scratch, kNoAllocScratchBytes);
 }
 
 ddg_status ddg_beam_search(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
@@ -1645,6 +1737,15 @@ ddg_status ddg_insert_search(ddg_index* idx, const uint32_t* ids, uint64_t m, ui
     std::memcpy(vis_dists + i * vcap, vd.data() + i * cap, vis_len[i] * 4);
   }
   return DDG_OK;
+}
+
+// Test hook (not part of include/ddg.h): device bytes of the handle's own
+// scratch buffers (the allocation-free entry point must leave them at zero).
+uint64_t ddg_internal_buffer_bytes(const ddg_index* idx) {
+  uint64_t t = 0;
+  if (idx)
+    for (const auto& b : idx->s.b) t += b.p ? b.cap : 0;
+  return t;
 }
 
 // Test hook (not part of include/ddg.h): records a search launch shape for

@@ -140,6 +140,22 @@ ddg_status ddg_search_device(ddg_index* idx, const void* queries, uint64_t nq, c
                              const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids, float* out_dists,
                              uint64_t* dist_cmps, uint32_t* visited);
 
+/* ddg_search_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * checks and results as ddg_search_device; the search's status words and
+ * its global visited-filter slices (beams above 90) live in a fixed 64 MiB
+ * device array that the library holds per GPU until the process ends, so
+ * large beams run with as many resident waves as their filter slices fit
+ * there, and a beam whose one block of filters exceeds it is rejected
+ * (DDG_EINVAL).  A reduced traversal copy must exist already
+ * (ddg_prepare_traversal).  Every call on a GPU shares that array: calls
+ * that overlap on one GPU race. */
+ddg_status ddg_search_device_noalloc(ddg_index* idx, const void* queries, uint64_t nq, const uint32_t* query_ids,
+                                     const uint32_t* starts, uint32_t nstarts, const ddg_query_params* qp,
+                                     const uint32_t* per_query_beam, uint32_t beam_max, uint32_t* out_ids,
+                                     float* out_dists, uint64_t* dist_cmps, uint32_t* visited);
+
 /* beam_search (beamSearch.h:41-54 signature, 50-184 algorithm) for nq points
  * in one launch, with everything the reference returns,
  * ((frontier, visited), dist_cmps): query i's final frontier (frontier_len[i]
