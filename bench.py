@@ -285,7 +285,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="headline workload only")
     ap.add_argument("--modes", default="native,bf16,fp8")
+    ap.add_argument("--workload", default="mips", choices=["mips", "u8", "i8", "s"],
+                    help="headline workload (mips: the 10M x 200 float32 set; the integer and 100K sets search "
+                         "in their native arithmetic only; used for per-workload profiles)")
     ap.add_argument("--cpu-build-n", type=int, default=200_000)
+    ap.add_argument("--only-gt", action="store_true",
+                    help="only time the FP64 / int8 MFMA ground truth (top-100) of the 10M x 200 float32 and "
+                         "10M x 128 uint8 workloads and exit (for matrix-core counter profiles)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (calibrates the FETCH_SIZE / "
                          "WRITE_SIZE counters against known bytes) and exit")
@@ -307,12 +313,30 @@ def main():
         print(json.dumps({"copy_probe_bytes": nbytes, "launches": 5}), flush=True)
         return
 
-    modes = args.modes.split(",")
-    head, X, Q, w, idx, G, gi, gd = run_workload(torch, "mips", modes, args.steps, args.warmup)
+    if args.only_gt:
+        from distributed_diskann_amd import Index, datasets
+        res = {}
+        for name in ("mips", "u8"):
+            X, Q, w = datasets.generate(name)
+            idx = Index(X, w.d, w.metric, 8)
+            for k in (10, GT_K):
+                idx.ground_truth(Q, k)
+                ms = idx.profile()["dense_ms"]
+                flops = 2.0 * X.shape[0] * Q.shape[0] * w.d
+                res[f"{name}_k{k}"] = {"ms": ms, "T_ops_per_s": flops / (ms / 1000.0) / 1e12}
+                log(f"[bench] ground truth {name} k={k}: {ms:.1f} ms, {flops / (ms / 1000.0) / 1e12:.1f} T ops/s")
+            idx.close()
+            del X, Q
+        print(json.dumps({"ground_truth": res}), flush=True)
+        return
+
+    modes = args.modes.split(",") if args.workload in ("mips", "s") else ["native"]
+    head, X, Q, w, idx, G, gi, gd = run_workload(torch, args.workload, modes, args.steps, args.warmup)
     low = {m: r for m, r in head["modes"].items() if m != "native" and r["recall"] >= 0.95}
-    best = max(low, key=lambda m: low[m]["qps"]) if low else "native"
+    best = max(low, key=lambda m: low[m]["qps"]) if low else ("native" if "native" in head["modes"] else modes[0])
     top = head["modes"][best]
     fp32 = head["modes"].get("native")
+    X_n = X.shape[0]
     idx.close()
     del idx
     torch.cuda.empty_cache()
@@ -323,7 +347,7 @@ def main():
     del X, G
 
     configs = {}
-    if not args.no_configs:
+    if not args.no_configs and args.workload == "mips":
         for name, extra in (("s", per_query_beams(torch, max(1, args.steps // 4))), ("i8", None), ("u8", None)):
             r, *rest = run_workload(torch, name, ["native"], max(1, args.steps // 2), 1, extra)
             rest[3].close()
@@ -331,20 +355,20 @@ def main():
             torch.cuda.empty_cache()
             configs[{"s": "100Kx20_f32", "i8": "1Mx100_i8", "u8": "10Mx128_u8"}[name]] = r
 
-    key = f"mips {best} Ls={top['Ls']}"
+    key = f"{args.workload} {best} Ls={top['Ls']}"
     traffic, traffic_src = profiled_traffic(key)
-    kname = f"k_search<float,mips,{best}>"
+    kname = f"k_search<{w.dtype},{w.metric},{best}>"
     out = {
         "metric": METRIC, "value": top["qps"], "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": top["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if best == "native" else f"{'bf16' if best == 'bf16' else 'fp8e4m3'} rows, f32 accumulate, "
-                                                "f32 re-rank",
-        "data": f"synthetic: {head['workload']}; 5000-component mixture seed {w.base_seed}, shifted queries seed "
-                f"{w.query_seed}",
-        "config": {"workload": f"10Mx200 f32 MIPS mixture, R=64 L=128 alpha=1.0, k=10, 10000 queries, "
-                               f"{best} traversal + FP32 re-rank of {RERANK}, Ls={top['Ls']}",
-                   "n": 10_000_000, "d": 200, "nq": 10_000, "R": 64, "L": 128, "alpha": 1.0, "k": 10,
+        "dtype": ({"float32": "f32", "uint8": "u8 (int32 accumulate)", "int8": "i8 (int32 accumulate)"}[w.dtype]
+                  if best == "native" else
+                  f"{'bf16' if best == 'bf16' else 'fp8e4m3'} rows, f32 accumulate, f32 re-rank"),
+        "data": f"synthetic: {head['workload']}; base seed {w.base_seed}, query seed {w.query_seed}",
+        "config": {"workload": f"{head['workload']}, {best} traversal" +
+                               (f" + FP32 re-rank of {RERANK}" if best != "native" else "") + f", Ls={top['Ls']}",
+                   "n": int(X_n), "d": w.d, "nq": int(Q.shape[0]), "R": w.R, "L": w.L, "alpha": w.alpha, "k": 10,
                    "mode": best, "Ls": top["Ls"]},
         "recall": top["recall"],
         "build_seconds": head["build_seconds"],
@@ -356,8 +380,8 @@ def main():
                      "frac": top["hbm_frac"], "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": kname, "kernel_avg_ms": top["kernel_ms"],
                      "alg_bytes_per_launch": top["alg_bytes_per_launch"],
-                     "note": "10M x 200 rows: the traversed copy and the 2.6 GB graph are far above the 256 MB "
-                             "Infinity Cache"},
+                     "note": (f"{X_n} rows: the traversed copy and the graph are far above the 256 MB Infinity "
+                              "Cache" if X_n >= 1_000_000 else "the working set is cache-resident")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

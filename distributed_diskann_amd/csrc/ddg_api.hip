@@ -1852,6 +1852,40 @@ ddg_status ddg_internal_group_pairs(ddg_index* idx, const uint32_t* keys, const 
   return DDG_OK;
 }
 
+ddg_status ddg_quantized_mips_distances(ddg_dtype code_dtype, uint32_t d, float max_coord, float min_coord,
+                                        const void* queries, int queries_quantized, uint64_t nq, uint32_t q_stride,
+                                        const void* codes, uint64_t n, uint32_t c_stride, float* out) {
+  ddg_index* idx = nullptr;
+  if (code_dtype != DDG_U8 && code_dtype != DDG_I8) return fail(idx, DDG_EINVAL, "codes must be uint8 or int8");
+  if (d == 0 || q_stride < d || c_stride < d) return fail(idx, DDG_EINVAL, "bad dimension or stride");
+  if (nq && n && (!queries || !codes || !out)) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (nq == 0 || n == 0) return DDG_OK;
+  if (nq > 65535) return fail(idx, DDG_EINVAL, "at most 65535 queries per call");
+  if (d > 16384) return fail(idx, DDG_EINVAL, "d above 16384 is not supported");
+  // mips_point.h:114-119: maxval = (float)(T)(2^(8 sizeof T) - 1), in float
+  const float maxval = code_dtype == DDG_U8 ? (float)(uint8_t)255u : (float)(int8_t)(uint8_t)255u;
+  const float delta = max_coord - min_coord;
+  const float mult = delta / maxval;
+  const size_t qbytes = nq * (size_t)q_stride * (queries_quantized ? 1 : 4), cbytes = n * (size_t)c_stride;
+  void *dq = nullptr, *dc = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&dq, qbytes);
+  if (e == hipSuccess) e = hipMalloc(&dc, cbytes);
+  if (e == hipSuccess) e = hipMalloc(&dout, nq * n * 4);
+  if (e == hipSuccess) e = hipMemcpy(dq, queries, qbytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dc, codes, cbytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_quantized_mips(code_dtype, dq, queries_quantized ? 1 : 0, nq, q_stride, dc, n, c_stride, d, mult,
+                              min_coord, (float*)dout, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, nq * n * 4, hipMemcpyDeviceToHost);
+  if (dq) (void)hipFree(dq);
+  if (dc) (void)hipFree(dc);
+  if (dout) (void)hipFree(dout);
+  if (e != hipSuccess)
+    return fail(idx, e == hipErrorOutOfMemory ? DDG_EOOM : DDG_EDEVICE,
+                std::string("quantized inner products: ") + hipGetErrorString(e));
+  return DDG_OK;
+}
+
 ddg_status ddg_ground_truth(ddg_index* idx, const void* queries, uint64_t nq, uint32_t k, uint32_t* ids,
                             double* dists) {
   if (!idx || (nq && (!queries || !ids))) return fail(idx, DDG_EINVAL, "NULL argument");

@@ -142,6 +142,12 @@ hipError_t launch_copy_rows(uint32_t* graph, uint32_t R1, const uint32_t* ids, u
                             int to_graph, hipStream_t s);
 hipError_t launch_pad_rows(void* rows, uint64_t n, uint32_t d_bytes, uint32_t row_bytes, hipStream_t s);
 
+// quant.hip: Quantized_Mips_Point distances of nq queries (float, or codes when
+// qq) to n code rows, out nq x n; strides in elements
+hipError_t launch_quantized_mips(int dtype, const void* queries, int qq, uint64_t nq, uint32_t qstride,
+                                 const void* codes, uint64_t n, uint32_t cstride, uint32_t d, float mult, float minc,
+                                 float* out, hipStream_t s);
+
 // dense.hip
 hipError_t launch_ground_truth(int dtype, int metric, const void* base, uint64_t n, const void* queries,
                                uint64_t nq, uint32_t d, uint32_t ad, uint32_t k, uint32_t* ids, double* dists,

@@ -441,6 +441,24 @@ def beam_search(p, G: Graph, Points: PointRange, starting_point, QP: QueryParams
     return (fr, vis), cmps
 
 
+def quantized_mips_distances(codes: np.ndarray, queries: np.ndarray, max_coord: float, min_coord: float):
+    """Quantized_Mips_Point<T>::distance (mips_point.h:111-187) of every
+    (query, code row) pair on the GPU, nq x n: float32 queries are
+    Mips_Point<float> queries, uint8 / int8 ones are quantized queries."""
+    codes = np.ascontiguousarray(codes)
+    queries = np.ascontiguousarray(queries)
+    if codes.dtype.name not in ("uint8", "int8") or codes.ndim != 2:
+        raise ValueError("codes must be a 2-D uint8 or int8 array")
+    qq = queries.dtype != np.float32
+    if (qq and queries.dtype != codes.dtype) or queries.ndim != 2 or queries.shape[1] < codes.shape[1]:
+        raise ValueError("queries must be float32 or of the codes' dtype, with at least d columns")
+    out = np.empty((queries.shape[0], codes.shape[0]), np.float32)
+    check(lib().ddg_quantized_mips_distances(DTYPES[codes.dtype.name], codes.shape[1], max_coord, min_coord,
+                                             _c(queries), int(qq), queries.shape[0], queries.shape[1], _c(codes),
+                                             codes.shape[0], codes.shape[1], _c(out)))
+    return out
+
+
 def recall_with_ties(found: np.ndarray, gt_ids: np.ndarray, gt_dists: np.ndarray, k: int) -> float:
     """check_nn_recall.h:86-109: the first k ground-truth ids plus every later
     one at the k-th distance count as correct answers."""

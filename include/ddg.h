@@ -195,6 +195,20 @@ ddg_status ddg_insert_search(ddg_index* idx, const uint32_t* ids, uint64_t m, ui
 ddg_status ddg_ground_truth(ddg_index* idx, const void* queries /* nq*aligned_d */, uint64_t nq, uint32_t k,
                             uint32_t* ids, double* dists);
 
+/* Quantized_Mips_Point<T>::distance (mips_point.h:111-187) of nq queries to
+ * n code rows (T = uint8 or int8 by code_dtype), in the reference's
+ * arithmetic: code c decodes to (float)c * mult + min_coord with
+ * mult = (max_coord - min_coord) / maxval, maxval = (float)(T)255 (255 for
+ * uint8; the cast makes it -1 for int8); the products with the query (float
+ * values, or codes decoded the same way when queries_quantized != 0) are
+ * added in dimension order; the sum is returned unnegated, as the reference
+ * returns it.  Strides in elements; out is nq x n (row-major); nq <= 65535.
+ * Host buffers; runs on the current device.  The reference's drivers never
+ * instantiate this point type, and no search here traverses with it. */
+ddg_status ddg_quantized_mips_distances(ddg_dtype code_dtype, uint32_t d, float max_coord, float min_coord,
+                                        const void* queries, int queries_quantized, uint64_t nq, uint32_t q_stride,
+                                        const void* codes, uint64_t n, uint32_t c_stride, float* out);
+
 /* index.h:133-136: sort every adjacency list by (distance to its owner, id). */
 ddg_status ddg_sort_rows(ddg_index* idx);
 

@@ -46,6 +46,9 @@
 //                                                 without the final sort
 //   ground_truth      (sklearn brute force in the  numpy float64 brute force
 //                     reference's dataset tools)
+//   quantized_mips    mips_point.h:111-187         golden vectors from oracle/_ref
+//                                                 (the reference's own
+//                                                 Quantized_Mips_Point::distance)
 //
 // Tie mode: ORC_TIE_REFERENCE sorts prune candidates and the final rows with
 // std::sort by distance only (index.h:78-79, 133-136), exactly as the
@@ -498,6 +501,32 @@ static void sort_rows(const Points& P, GraphView& G, int tie_mode, int nthreads)
   });
 }
 
+// Quantized_Mips_Point<T>::distance (mips_point.h:111-141, 160-162): code c
+// decodes to (float)c * mult + min_coord with mult = (max - min) / maxval and
+// maxval = (float)(T)(2^(8 sizeof T) - 1) (255 for uint8; the cast makes it
+// -1 for int8); the products with the query (float, or decoded the same way
+// when qq) are added in dimension order, and the sum is returned unnegated.
+// out is nq x n.
+template <class T>
+static void qmips(uint32_t d, float mx, float mn, const void* queries, int qq, uint64_t nq, const T* codes,
+                  uint64_t n, float* out) {
+  const float maxval = static_cast<float>(static_cast<T>((((size_t)1) << (sizeof(T) * 8)) - 1));
+  const float delta = mx - mn;
+  const float mult = delta / maxval;
+  for (uint64_t i = 0; i < nq; i++)
+    for (uint64_t j = 0; j < n; j++) {
+      const T* p = codes + j * d;
+      float r = 0;
+      for (uint32_t k = 0; k < d; k++) {
+        const float dc = static_cast<float>(p[k]) * mult + mn;
+        const float qk = qq ? static_cast<float>(((const T*)queries)[i * d + k]) * mult + mn
+                            : ((const float*)queries)[i * d + k];
+        r += dc * qk;
+      }
+      out[i * n + j] = r;
+    }
+}
+
 }  // namespace orc
 
 // ================================================================ C entry points
@@ -681,6 +710,14 @@ int orc_build(const orc_points* P, orc_graph* G, const orc_build_params* bp, int
 }
 
 void orc_random_permutation(uint64_t n, int32_t* out) { orc::random_permutation(n, out); }
+
+int orc_quantized_mips(int dtype, uint32_t d, float mx, float mn, const void* queries, int qq, uint64_t nq,
+                       const void* codes, uint64_t n, float* out) {
+  if (dtype == orc::U8) orc::qmips<uint8_t>(d, mx, mn, queries, qq, nq, (const uint8_t*)codes, n, out);
+  else if (dtype == orc::I8) orc::qmips<int8_t>(d, mx, mn, queries, qq, nq, (const int8_t*)codes, n, out);
+  else return -1;
+  return 0;
+}
 
 int orc_sort_rows(const orc_points* P, orc_graph* G, int tie_mode, int nthreads) {
   orc::Points pp = mk(P);

@@ -101,6 +101,8 @@ def load(path):
                                       C.c_int]
     L.orc_distances.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint32,
                                    C.c_uint32, C.c_void_p]
+    L.orc_quantized_mips.argtypes = [C.c_int, C.c_uint32, C.c_float, C.c_float, C.c_void_p, C.c_int, C.c_uint64,
+                                     C.c_void_p, C.c_uint64, C.c_void_p]
     return L
 
 
@@ -259,3 +261,18 @@ def ground_truth(P: Points, Q: np.ndarray, k, nthreads=0):
     assert lib().orc_ground_truth(C.byref(P.s), Q.ctypes.data, nq, k, ids.ctypes.data, d.ctypes.data,
                                   nthreads) == 0
     return ids, d
+
+
+def quantized_mips(codes: np.ndarray, queries: np.ndarray, max_coord, min_coord) -> np.ndarray:
+    """Quantized_Mips_Point<T>::distance of every (query, code) pair, nq x n;
+    float32 queries are Mips_Point<float> queries, code-typed ones are
+    quantized queries."""
+    codes = np.ascontiguousarray(codes)
+    queries = np.ascontiguousarray(queries)
+    assert codes.shape[1] == queries.shape[1]
+    qq = int(queries.dtype != np.float32)
+    out = np.empty((queries.shape[0], codes.shape[0]), np.float32)
+    r = lib().orc_quantized_mips(DTYPES[codes.dtype.name], codes.shape[1], max_coord, min_coord, queries.ctypes.data,
+                                 qq, queries.shape[0], codes.ctypes.data, codes.shape[0], out.ctypes.data)
+    assert r == 0
+    return out

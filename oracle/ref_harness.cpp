@@ -19,6 +19,11 @@
 //   groupby <pairs> <out>                      parlay::group_by_key
 //   perm <n> <out>                             parlay::random_permutation<int>(n)
 //                                              (batch_insert's random_order)
+//   qmips <dtype> <codes> <queries> <qq> <max> <min> <out>
+//                                              f32 Quantized_Mips_Point<T>(codes[j])
+//                                              .distance(query i) for every (i, j),
+//                                              the query a Mips_Point<float> (qq 0)
+//                                              or a Quantized_Mips_Point<T> (qq 1)
 // Vector files use the reference format [u32 n][u32 d][n*d T]; <ids> and
 // <pairs> are raw u32 arrays.
 
@@ -119,6 +124,49 @@ static int run_dist(int argc, char** argv) {
   return 0;
 }
 
+// [u32 n][u32 d][n*d T] -> rows (unpadded), n, d
+template <class T>
+static std::vector<T> read_vecs(const char* fn, uint32_t& n, uint32_t& d) {
+  std::ifstream f(fn, std::ios::binary);
+  f.read((char*)&n, 4);
+  f.read((char*)&d, 4);
+  std::vector<T> v((size_t)n * d);
+  f.read((char*)v.data(), v.size() * sizeof(T));
+  return v;
+}
+
+template <class T>
+static int run_qmips(char** argv) {
+  uint32_t n, d, nq, dq;
+  std::vector<T> codes = read_vecs<T>(argv[3], n, d);
+  const bool qq = atoi(argv[5]) != 0;
+  const float mx = (float)atof(argv[6]), mn = (float)atof(argv[7]);
+  FILE* out = fopen(argv[8], "wb");
+  if (qq) {
+    std::vector<T> q = read_vecs<T>(argv[4], nq, dq);
+    for (uint32_t i = 0; i < nq; i++) {
+      Quantized_Mips_Point<T> qp(q.data() + (size_t)i * d, d, d, i, mx, mn);
+      for (uint32_t j = 0; j < n; j++) {
+        Quantized_Mips_Point<T> cp(codes.data() + (size_t)j * d, d, d, j, mx, mn);
+        float r = cp.distance(qp);
+        fwrite(&r, 4, 1, out);
+      }
+    }
+  } else {
+    std::vector<float> q = read_vecs<float>(argv[4], nq, dq);
+    for (uint32_t i = 0; i < nq; i++) {
+      Mips_Point<float> qp(q.data() + (size_t)i * d, d, d, i);
+      for (uint32_t j = 0; j < n; j++) {
+        Quantized_Mips_Point<T> cp(codes.data() + (size_t)j * d, d, d, j, mx, mn);
+        float r = cp.distance(qp);
+        fwrite(&r, 4, 1, out);
+      }
+    }
+  }
+  fclose(out);
+  return 0;
+}
+
 template <template <class> class PT>
 static int dispatch_dtype(const std::string& mode, const std::string& dtype, int argc, char** argv) {
   if (dtype == "uint8") {
@@ -172,6 +220,12 @@ int main(int argc, char** argv) {
     }
     fclose(out);
     return 0;
+  }
+  if (mode == "qmips") {
+    std::string dtype = argv[2];
+    if (dtype == "uint8") return run_qmips<uint8_t>(argv);
+    if (dtype == "int8") return run_qmips<int8_t>(argv);
+    return 2;
   }
   if (mode == "search" || mode == "insert" || mode == "dist") {
     std::string dtype = argv[2], metric = argv[3];

@@ -21,6 +21,9 @@ from its headers by oracle/Makefile into oracle/_ref/ref_harness:
                   workload (seed 0 base, seed 1 queries)
   perm.npz        parlay::random_permutation<int>(n), batch_insert's
                   random_order (`make_golden.py perm` writes only this one)
+  qmips.npz       Quantized_Mips_Point<T>::distance over uint8 / int8 codes
+                  against float queries (Mips_Point<float>) and against
+                  quantized queries (`make_golden.py qmips` writes only this one)
 """
 from __future__ import annotations
 
@@ -101,12 +104,43 @@ def make_perm(tmp):
     np.savez_compressed(os.path.join(HERE, "perm.npz"), **perm)
 
 
+def make_qmips(tmp):
+    """Quantized_Mips_Point (mips_point.h:111-187): codes decoded as
+    code * (max - min) / maxval + min, summed in order against the query."""
+    rng = np.random.default_rng(77)
+    out = {}
+    for dt, d, mx, mn in (("uint8", 100, 3.75, -2.125), ("int8", 100, 1.5, -1.25), ("uint8", 17, 0.3, 0.1),
+                          ("int8", 33, 250.0, -0.5)):
+        npd = O.NP_DTYPES[dt]
+        lo, hi = (0, 256) if dt == "uint8" else (-128, 128)
+        codes = rng.integers(lo, hi, (64, d)).astype(npd)
+        codes[0] = lo
+        codes[1] = hi - 1
+        qf = (rng.standard_normal((16, d)) * rng.uniform(0.1, 10, (16, 1))).astype(np.float32)
+        qc = rng.integers(lo, hi, (16, d)).astype(npd)
+        fc, fq, fo = (os.path.join(tmp, x) for x in ("qc", "qq", "qo"))
+        formats.write_vectors(fc, codes)
+        key = f"{dt}_{d}"
+        for qq, Q in ((0, qf), (1, qc)):
+            formats.write_vectors(fq, Q)
+            run("qmips", dt, fc, fq, qq, repr(mx), repr(mn), fo)
+            out[f"{key}_q{qq}"] = Q
+            out[f"{key}_out{qq}"] = np.fromfile(fo, np.float32).reshape(16, 64)
+        out[f"{key}_codes"] = codes
+        out[f"{key}_max"] = np.float32(mx)
+        out[f"{key}_min"] = np.float32(mn)
+    np.savez_compressed(os.path.join(HERE, "qmips.npz"), **out)
+
+
 def main():
     if not (os.path.exists(HARNESS) and os.path.isdir(REF)):
         sys.exit("needs the reference tree and oracle/_ref/ref_harness (make -C oracle)")
     tmp = tempfile.mkdtemp()
     if sys.argv[1:] == ["perm"]:  # only the permutation fixture
         make_perm(tmp)
+        return
+    if sys.argv[1:] == ["qmips"]:  # only the quantized inner-product fixture
+        make_qmips(tmp)
         return
 
     # hash64_2

@@ -200,3 +200,30 @@ def test_reduced_errors():
     with pytest.raises(DdgError) as e:
         idx.search(Q, 10, 20, 1.35, trav="fp8", rerank=5)
     assert e.value.status == DDG_EINVAL
+
+
+@pytest.mark.parametrize("case", ["uint8_100", "int8_100", "uint8_17", "int8_33"])
+@pytest.mark.parametrize("qq", [0, 1])
+def test_quantized_mips_golden(case, qq):
+    """Quantized_Mips_Point::distance on the GPU equals the reference's own
+    (golden vectors from its compiled mips_point.h) bit for bit."""
+    print(f"[gpu] quantized inner products {case} quantized_query={qq}", flush=True)
+    from conftest import load_golden
+    from distributed_diskann_amd.index import quantized_mips_distances
+    z = load_golden("qmips")
+    got = quantized_mips_distances(z[case + "_codes"], z[f"{case}_q{qq}"], float(z[case + "_max"]),
+                                   float(z[case + "_min"]))
+    assert np.array_equal(got.view(np.uint32), z[f"{case}_out{qq}"].view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", ["uint8", "int8"])
+def test_quantized_mips_vs_oracle(oracle, dtype):
+    print(f"[gpu] quantized inner products {dtype} 300 x 5000 d=128", flush=True)
+    from distributed_diskann_amd.index import quantized_mips_distances
+    rng = np.random.default_rng(41)
+    lo, hi = (0, 256) if dtype == "uint8" else (-128, 128)
+    codes = rng.integers(lo, hi, (5000, 128)).astype(dtype)
+    for Q in (rng.standard_normal((300, 128)).astype(np.float32), rng.integers(lo, hi, (300, 128)).astype(dtype)):
+        got = quantized_mips_distances(codes, Q, 0.75, -0.5)
+        want = oracle.quantized_mips(codes, Q, 0.75, -0.5)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))

@@ -343,3 +343,22 @@ def test_ground_truth_matches_reference_file(oracle, xs):
     # the reference file stores non-squared distances
     assert np.allclose(np.sqrt(d), xs["gt_d"][:200, :10], rtol=1e-5)
     assert (ids == xs["gt_ids"][:200, :10]).mean() > 0.999
+
+
+@pytest.mark.parametrize("case", ["uint8_100", "int8_100", "uint8_17", "int8_33"])
+@pytest.mark.parametrize("qq", [0, 1])
+def test_quantized_mips_golden_bits(oracle, case, qq):
+    # the reference's own Quantized_Mips_Point::distance (mips_point.h:111-187)
+    z = load_golden("qmips")
+    got = oracle.quantized_mips(z[case + "_codes"], z[f"{case}_q{qq}"], float(z[case + "_max"]),
+                                float(z[case + "_min"]))
+    assert np.array_equal(got.view(np.uint32), z[f"{case}_out{qq}"].view(np.uint32))
+
+
+def test_quantized_mips_decoding(oracle):
+    # uint8 code c decodes to c * (max - min) / 255 + min, so a code row of
+    # 255s against a one-hot query is max, of 0s is min; the sum is not negated
+    codes = np.array([[255, 0], [0, 0]], np.uint8)
+    q = np.array([[1.0, 0.0]], np.float32)
+    got = oracle.quantized_mips(codes, q, 2.0, -1.0)
+    assert np.allclose(got, [[2.0, -1.0]], atol=1e-6)
