@@ -166,7 +166,9 @@ static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
   a.trav_chunks = a.trav_stride / 16;
   // a reduced row is split over the fewest lanes (1, 2, 4 or 8) that leave
   // each lane at most one load batch of 8 chunks: one memory round trip per
-  // distance
+  // distance.  (A fixed pairwise tree of 8 blocks that would let one row be
+  // split wider when an expansion has few candidates was measured slower:
+  // 10M x 200 MIPS, FP8 at Ls=250, 4.04 ms against 3.88 ms.)
   while (a.trav_split < 8 && (a.trav_chunks + a.trav_split - 1) / a.trav_split > 8) a.trav_split *= 2;
   a.trav_cps = (a.trav_chunks + a.trav_split - 1) / a.trav_split;
   a.trav_rows = idx->trav_rows[trav];
@@ -192,6 +194,25 @@ static bool global_table(uint32_t bits) {
 }
 
 static uint32_t tune_key(int trav, uint32_t bits) { return ((uint32_t)trav << 8) | bits; }
+
+// Filter sizes (log2 slots) that keep a written-slot bitmap with a global
+// filter (native traversal; the reduced-precision kernels are compiled
+// without it): 2^12 .. 2^13 slots.  Larger filters with a bitmap were
+// measured slower (10M x 200 MIPS, native at Ls=275, 2^15 slots: 9.36 ms
+// against 8.72 ms without): the search waits on latency, not on the
+// filter's traffic, and the bitmap costs LDS and occupancy.
+// DDG_WBITS="lo-hi" (or "0" for none) overrides the range for measurement;
+// it is read at every launch and results are the same either way.
+static std::pair<uint32_t, uint32_t> wbits_range() {
+  std::pair<uint32_t, uint32_t> v{12u, 13u};
+  const char* e = getenv("DDG_WBITS");
+  if (e) {
+    unsigned lo = 0, hi = 0;
+    if (sscanf(e, "%u-%u", &lo, &hi) == 2 && lo <= hi && hi <= 24) v = {lo, hi};
+    else if (e[0] == '0') v = {1u, 0u};
+  }
+  return v;
+}
 
 // Fills the LDS layout and launch shape of a search; a.gtable_words > 0 asks
 // the caller for a global filter pool of blocks * waves_per_block slices.
@@ -238,12 +259,13 @@ static bool search_layout_cfg(const ddg_index* idx, uint32_t beam_max, uint32_t 
   a.off_aux = off; off += rcap * 8;
   a.off_bkt = off; off += 1024;
   // with a global filter, a bitmap of the slots the current query has
-  // written (512 B or 1 KB): a never-written slot needs no global read, and
-  // the slice needs no clearing per query (13 % less fetched and 29 % less
-  // written at the 10M uint8 workload, Ls=140).  Below 2^12 slots a query
-  // writes most slots anyway (measured 2 % slower at Ls=80); above 2^13 its
-  // LDS costs more occupancy than it saves.
-  a.wbits = gtab && bits >= 12 && bits <= 13 && trav == DDG_TRAV_NATIVE ? 1u : 0u;
+  // written (2^bits / 8 bytes): a never-written slot needs no global read,
+  // and the slice needs no clearing per query (13 % less fetched and 29 %
+  // less written at the 10M uint8 workload, Ls=140).  Below 2^12 slots a
+  // query writes most slots anyway (measured 2 % slower at Ls=80).  The
+  // range is wbits_range().
+  const auto wr = wbits_range();
+  a.wbits = gtab && bits >= wr.first && bits <= wr.second && trav == DDG_TRAV_NATIVE ? 1u : 0u;
   a.off_wbits = off;
   if (a.wbits) off += std::max<uint32_t>(16u, (1u << bits) / 8u);
   // rows of more than one load batch (8 chunks of 16 B) can be gathered into

@@ -97,6 +97,31 @@ def test_search_staged_rows(oracle, dtype, metric, d, stage):
     assert np.array_equal(base["native"][0], oi) and np.array_equal(base["native"][2], oc)
 
 
+@pytest.mark.parametrize("stage", [3, 8])
+def test_search_staged_rows_low_degree(stage):
+    """The 10M x 200 MIPS workload's distribution at 30,000 points (alpha
+    1.0, R=64: about 5 neighbors per row, so most expansions have a few
+    candidates): staged rows (one lane per row) give the same bits, counts and
+    visits as rows split over lanes."""
+    print(f"[gpu] staged rows, MIPS mixture, low degree, stage={stage}", flush=True)
+    import ctypes as C
+    from distributed_diskann_amd import Index, datasets
+    from distributed_diskann_amd._lib import lib
+    X, Q, w = datasets.generate("mips", n=30_000, nq=400)
+    idx = Index(X, w.d, w.metric, w.R)
+    idx.build(R=w.R, L=w.L, alpha=w.alpha)
+    base = {m: idx.search(Q, 10, 100, 1.35, trav=m, rerank=50) for m in ("bf16", "fp8")}
+    f = lib().ddg_internal_set_shape
+    f.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32]
+    f.restype = C.c_int
+    for t, m in ((1, "bf16"), (2, "fp8")):
+        assert f(idx.h, t, 100, 1, 4, 2, stage) == 0
+        ids, dists, cmps, vis = idx.search(Q, 10, 100, 1.35, trav=m, rerank=50)
+        b = base[m]
+        assert np.array_equal(ids, b[0]) and np.array_equal(dists.view(np.uint32), b[1].view(np.uint32))
+        assert np.array_equal(cmps, b[2]) and np.array_equal(vis, b[3])
+
+
 def test_search_per_query_beam(oracle, xs):
     print("[gpu] per-query beam", flush=True)
     rng = np.random.default_rng(7)
