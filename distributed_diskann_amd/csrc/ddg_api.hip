@@ -165,11 +165,14 @@ static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
   a.trav_stride = trav_stride_for(idx, trav);
   a.trav_chunks = a.trav_stride / 16;
   // a reduced row is split over the fewest lanes (1, 2, 4 or 8) that leave
-  // each lane at most one load batch of 8 chunks: one memory round trip per
-  // distance.  (A fixed pairwise tree of 8 blocks that would let one row be
-  // split wider when an expansion has few candidates was measured slower:
-  // 10M x 200 MIPS, FP8 at Ls=250, 4.04 ms against 3.88 ms.)
-  while (a.trav_split < 8 && (a.trav_chunks + a.trav_split - 1) / a.trav_split > 8) a.trav_split *= 2;
+  // each lane at most kLaneChunks chunks (rows above 512 bytes: as few as
+  // possible over 8 lanes).  Expansions have few candidates on sparse graphs,
+  // so narrow lane shares cost less arithmetic than full 64-lane batches:
+  // 10M x 200 MIPS at Ls=250, FP8 split 4 ways 3.75 ms against 3.86 ms for
+  // 2 ways, BF16 8 ways 4.42 ms against 4.67 ms for 4 ways.  (A fixed pairwise
+  // tree of 8 blocks, which would let any expansion pick its own split, was
+  // measured slower: 4.04 ms.)
+  while (a.trav_split < 8 && (a.trav_chunks + a.trav_split - 1) / a.trav_split > kLaneChunks) a.trav_split *= 2;
   a.trav_cps = (a.trav_chunks + a.trav_split - 1) / a.trav_split;
   a.trav_rows = idx->trav_rows[trav];
   a.trav_scale = idx->trav_scale;

@@ -10,6 +10,12 @@
 namespace ddg {
 
 // ------------------------------------------------------------------ search
+
+// A lane's share of a reduced (BF16 / FP8) row in a search: at most
+// kLaneChunks 16-byte chunks for rows up to 512 bytes (split over up to 8
+// lanes).
+constexpr uint32_t kLaneChunks = 4;
+
 struct SearchArgs {
   const void* base;              // n x ad rows
   const uint32_t* graph;         // n x (R+1)
