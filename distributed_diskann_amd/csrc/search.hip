@@ -103,12 +103,15 @@ __device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t 
 // STAGE: candidate rows are gathered into LDS before their distances (a
 // launch choice for long rows; a separate instantiation, so the other
 // kernels do not carry its registers).  The native kernels are compiled for
-// at least 6 waves per SIMD (80 VGPRs): measured faster than the compiler's
-// own choice, which hoists every staged query chunk of a distance.  The BF16
-// kernels take 5 (96 VGPRs; 1.39 -> 1.23 ms on 100K x 20); FP8 keeps the
-// compiler's choice (no gain at 5).
+// at least 5 waves per SIMD (96 VGPRs; the compiler's own choice hoists
+// every staged query chunk of a distance).  At 6 waves (80 VGPRs) they
+// spill, and every spill reload is a memory load that waits for all loads in
+// flight (they retire in order), the prefetched ones included: 10M x 128
+// uint8 at Ls=140, 3.37 ms at 5 against 3.60 ms at 6; the 10M builds 9-14 %
+// faster.  The BF16 kernels take 5 (96 VGPRs; 1.39 -> 1.23 ms on 100K x 20,
+// the same as 4 at 10M); FP8 keeps the compiler's choice (no gain at 5).
 template <typename T, int METRIC, int TRAV, bool GTAB, bool STAGE>
-__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 ? 5 : 1))) void k_search(SearchArgs a) {
+__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 ? 5 : 1))) void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using P = TravDist<T, METRIC, TRAV>;
   const uint32_t lane = lane_id();
@@ -227,15 +230,18 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 
       const float cdist = __uint_as_float(ce.x);
       const uint32_t cid = ce.y & kIdMask;
       const uint32_t* row = a.graph + (uint64_t)cid * R1;
-      uint32_t deg, nb0;
       const bool use_pf_table = GTAB && pf_table && pf_id == cid;
-      if (pf_id == cid) {
-        deg = pf_deg;
-        nb0 = pf_nb;
-      } else {  // one round trip: the degree and the neighbor words together
-        nb0 = lane < a.R ? row[1 + lane] : 0u;
-        deg = row[0];
+      // the adjacency row is in pf_nb / pf_deg when the prediction holds;
+      // otherwise it is loaded into those registers (one round trip: degree
+      // and neighbor words together).  No copy is made of them here: a copy
+      // would wait for every load in flight, the table words loaded ahead
+      // included, instead of the filter's first use waiting for what it needs
+      if (pf_id != cid) {
+        pf_nb = lane < a.R ? row[1 + lane] : 0u;
+        pf_deg = row[0];
       }
+      const uint32_t nb0 = pf_nb;
+      const uint32_t deg = pf_deg;
       wave_sync();
       if (lane == 0) {
         ((uint32_t*)FR)[2 * pos + 1] = cid | kFlag;
@@ -279,7 +285,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 
       // ---- neighbor filter (beamSearch.h:125-131)
       const uint32_t m = deg < a.degree_limit ? deg : a.degree_limit;
       uint32_t nk = 0;
-      for (uint32_t b0 = 0; b0 < m; b0 += kWave) {
+      auto filter_batch = [&](uint32_t b0) __attribute__((always_inline)) {
         const uint32_t j = b0 + lane;
         const bool valid = j < m;
         const uint32_t av = valid ? (b0 == 0 ? nb0 : row[1 + j]) : 0u;
@@ -335,7 +341,12 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 6 : (TRAV == TRAV_BF16 
         if (keep) Kid[nk + mbcnt(km)] = av;
         nk += popc(km);
         wave_sync();
-      }
+      };
+      // the first 64 neighbors outside the loop: the compiler waits for
+      // every pending load at a loop header, and the first batch's table
+      // words may still be in flight
+      if (m > 0) filter_batch(0);
+      for (uint32_t b0 = kWave; b0 < m; b0 += kWave) filter_batch(b0);
       cmps += nk;
       // the predicted vertex's adjacency row, requested with the candidate
       // rows below: the wait for those rows covers it too (requested earlier,
