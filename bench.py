@@ -337,6 +337,18 @@ def main():
     top = head["modes"][best]
     fp32 = head["modes"].get("native")
     X_n = X.shape[0]
+    # the PCIe-inclusive rate: the same search through ddg_search with the
+    # queries and results in host memory (copied in and out around the
+    # launch); reported next to the device-resident value, never as it
+    rr = 0 if best == "native" else RERANK
+    idx.search(Q, 10, top["Ls"], 1.35, trav=best, rerank=rr)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        idx.search(Q, 10, top["Ls"], 1.35, trav=best, rerank=rr)
+    host_ms = (time.perf_counter() - t0) * 1000.0 / 3
+    host = {"qps": Q.shape[0] / (host_ms / 1000.0), "ms_per_call": host_ms,
+            "note": "ddg_search with host query and result buffers (PCIe copies included)"}
+    log(f"[bench] {args.workload} {best}: host buffers {host['qps']:.0f} QPS ({host_ms:.3f} ms per call)")
     idx.close()
     del idx
     torch.cuda.empty_cache()
@@ -374,6 +386,7 @@ def main():
         "build_seconds": head["build_seconds"],
         "fp32": None if fp32 is None else {"qps": fp32["qps"], "Ls": fp32["Ls"], "recall": fp32["recall"]},
         "speedup_vs_fp32": None if fp32 is None else top["qps"] / fp32["qps"],
+        "host_buffers": host,
         "headline": head,
         "configs": configs,
         "roofline": {"bound": "hbm", "achieved": top["alg_GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
