@@ -439,7 +439,14 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
     HIPCHK(launch_fill_u32(flags + F_WORK, 0u, 2, idx->stream));
     SearchArgs a{};
     Launch l{};
-    if (!search_layout(idx, L, 1, DDG_TRAV_NATIVE, a, l, false))
+    // DDG_BUILD_SHAPE="gtab,waves_per_block,blocks_per_cu,stage_rows" fixes
+    // the insert search's launch shape, for measurement (results are the
+    // same for every shape)
+    ddg_index::Tuned bcfg{};
+    const char* bshape = getenv("DDG_BUILD_SHAPE");
+    const bool pinned = bshape && sscanf(bshape, "%d,%u,%u,%u", &bcfg.gtab, &bcfg.wpb, &bcfg.bpc, &bcfg.stage) == 4;
+    if (pinned ? !search_layout_cfg(idx, L, 1, DDG_TRAV_NATIVE, &bcfg, a, l)
+               : !search_layout(idx, L, 1, DDG_TRAV_NATIVE, a, l, false))
       return fail(idx, DDG_EINVAL, "build beam L too large for LDS");
     a.base = idx->rows;
     a.graph = idx->graph;

@@ -370,14 +370,14 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 
           const uint32_t nr = nk - r0 < a.stage_rows ? nk - r0 : a.stage_rows;
           const uint32_t total = nr * C;
           for (uint32_t e0 = 0; e0 < total; e0 += kChunkBatch * kWave) {
+            // unpredicated loads (an element beyond the batch re-reads the
+            // last one): a conditionally assigned array is kept in scratch
             uint4 xs[kChunkBatch];
 #pragma unroll
             for (uint32_t u = 0; u < kChunkBatch; u++) {
-              const uint32_t e = e0 + u * kWave + lane;
-              if (e < total) {
-                const uint32_t r = e / C, c = e - r * C;
-                xs[u] = P::row_ptr(a, Kid[r0 + r])[c];
-              }
+              const uint32_t e = e0 + u * kWave + lane < total ? e0 + u * kWave + lane : total - 1u;
+              const uint32_t r = e / C, c = e - r * C;
+              xs[u] = P::row_ptr(a, Kid[r0 + r])[c];
             }
 #pragma unroll
             for (uint32_t u = 0; u < kChunkBatch; u++) {
