@@ -26,26 +26,28 @@ struct Buf {
 };
 
 struct Scratch {
-  Buf b[40];
+  Buf b[48];
 };
 
 enum BufId {
   B_QUERIES, B_STARTS, B_QBEAM, B_QIDS, B_OUT_IDS, B_OUT_D, B_CMPS, B_VISN, B_VIS_IDS, B_VIS_D, B_FLAGS,
   B_INSERTS, B_NEW_IDS, B_NEW_LEN, B_OFFS, B_KEYS, B_VALS, B_KEYS2, B_VALS2, B_TMP, B_GFLAGS, B_GOFFS, B_GKEYS,
   B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D, B_GTAB, B_BIGOFF, B_RPTS,
-  B_FR_IDS, B_FR_D, B_FR_LEN, B_ROW_IDS, B_ROWS, B_COUNT
+  B_FR_IDS, B_FR_D, B_FR_LEN, B_ROW_IDS, B_ROWS, B_GEPOCH, B_COUNT
 };
-static_assert(B_COUNT <= 40, "scratch slots");
+static_assert(B_COUNT <= 48, "scratch slots");
 
 // Device words of B_FLAGS (u32 index): [0] search work counter, [1] visited
 // list overflow, [4] error status (bit 0 degree overflow, bit 1 prune scratch
 // exhausted), [8..9] pair count (u64), [10..11] group count (u64), [12]
 // largest group, [16] prune work counter, [17] long-list count, [18]
-// long-list work counter, [20..21] long-list scratch handed out (u64).
+// long-list work counter, [20..21] long-list scratch handed out (u64),
+// [24..25] a build round's batch size, 0 when its insert search overflowed
+// (u64), [26..27] dist_cmps of the build's insert searches (u64).
 constexpr size_t kFlagBytes = 256;
 enum FlagWord : uint32_t {
   F_WORK = 0, F_OVERFLOW = 1, F_STATUS = 4, F_PAIRS = 8, F_GROUPS = 10, F_MAXG = 12, F_PWORK = 16, F_NBIG = 17,
-  F_PWORK2 = 18, F_GUSED = 20
+  F_PWORK2 = 18, F_GUSED = 20, F_ROUNDB = 24, F_CMPS = 26
 };
 
 }  // namespace
@@ -72,6 +74,14 @@ struct ddg_index {
   const Tuned* cfg_override = nullptr;  // set while the autotuner times a shape
   uint32_t num_cu = 256;
   unsigned char* noalloc_scratch = nullptr;  // this GPU's copy of `scratch` (null: its address was not found)
+  // what the global filter pool holds for epoch-tagged launches: the pool and
+  // epoch buffers it was initialised in, its slice size and how many slices
+  // are initialised (all 0xFFFFFFFF with epoch 0, or left by earlier
+  // epoch-tagged launches of this layout); null after any other use
+  const void* ep_pool = nullptr;
+  const void* ep_epochs = nullptr;
+  uint32_t ep_words = 0;
+  uint64_t ep_slices = 0;
   std::string noalloc_err;
   Scratch s;
   std::string err;
@@ -303,12 +313,58 @@ static bool search_layout_cfg(const ddg_index* idx, uint32_t beam_max, uint32_t 
   return true;
 }
 
-#define ENSURE_GTABLE(a, l)                                                                                 \
-  do {                                                                                                     \
-    if ((a).gtable_words) {                                                                                \
-      ENSURE(B_GTAB, (size_t)(l).blocks * (l).waves_per_block * (a).gtable_words * 4);                      \
-      (a).gtable = buf<uint32_t>(idx, B_GTAB);                                                             \
-    }                                                                                                      \
+// Bits an id below n takes (at least 1).
+static uint32_t id_bits_for(uint64_t n) {
+  uint32_t b = 1;
+  while (b < 64 && (1ull << b) < n) b++;
+  return b;
+}
+
+// The global filter pool of a launch (a.gtable_words > 0): one slice per
+// resident wave.  Without the written-slot bitmap, and when ids leave at
+// least two bits of a 32-bit word free, the slices are epoch-tagged: the
+// pool and its per-slice epochs are initialised once per slice size, and
+// later launches of that size continue their epochs (any other use of the
+// pool, or a new allocation, starts over).
+static ddg_status gtable_prepare(ddg_index* idx, SearchArgs& a, const Launch& l) {
+  a.gepoch = nullptr;
+  if (!a.gtable_words) return DDG_OK;
+  const uint64_t slices = (uint64_t)l.blocks * l.waves_per_block;
+  ENSURE(B_GTAB, (size_t)slices * a.gtable_words * 4);
+  a.gtable = buf<uint32_t>(idx, B_GTAB);
+  const uint32_t ib = id_bits_for(idx->n);
+  const char* ep_env = getenv("DDG_EPOCH");  // DDG_EPOCH=0: clear per query, for measurement
+  if (a.wbits || ib > 30 || (ep_env && ep_env[0] == '0')) {
+    idx->ep_pool = nullptr;  // this launch stores untagged ids
+    return DDG_OK;
+  }
+  ENSURE(B_GEPOCH, (size_t)slices * 4);
+  uint32_t* ep = buf<uint32_t>(idx, B_GEPOCH);
+  const bool same = idx->ep_pool == a.gtable && idx->ep_epochs == ep && idx->ep_words == a.gtable_words;
+  const uint64_t from = same ? std::min(idx->ep_slices, slices) : 0;
+  if (from < slices) {
+    HIPCHK(launch_fill_u32(a.gtable + from * a.gtable_words, 0xFFFFFFFFu, (slices - from) * a.gtable_words,
+                           idx->stream));
+    HIPCHK(launch_fill_u32(ep + from, 0u, slices - from, idx->stream));
+  }
+  idx->ep_pool = a.gtable;
+  idx->ep_epochs = ep;
+  idx->ep_words = a.gtable_words;
+  idx->ep_slices = std::max<uint64_t>(same ? idx->ep_slices : 0, slices);
+  a.gepoch = ep;
+  a.id_bits = ib;
+  a.epoch_end = (1u << (32 - ib)) - 1u;  // epoch_end << ib | id is never 0xFFFFFFFF below it
+  // DDG_EPOCH_END=<e> (e >= 2) shortens the epoch cycle, so that tests reach
+  // the wrap-around clear with few queries; results are the same
+  const char* ee = getenv("DDG_EPOCH_END");
+  if (ee && atoi(ee) >= 2) a.epoch_end = std::min<uint32_t>(a.epoch_end, (uint32_t)atoi(ee));
+  return DDG_OK;
+}
+
+#define ENSURE_GTABLE(a, l)                            \
+  do {                                                 \
+    ddg_status st_ = gtable_prepare(idx, (a), (l));    \
+    if (st_ != DDG_OK) return st_;                     \
   } while (0)
 
 // Launch tunables, read once from the environment (for measurement sweeps):
@@ -423,72 +479,86 @@ static uint32_t key_bits_for(uint64_t n) {
   return ((b + 7) / 8) * 8;
 }
 
-// beam search in build mode over pts[0..B) (device), visited lists to
-// B_VIS_IDS / B_VIS_D with stride *vcap, lengths to B_VISN.
+// Beam search in build mode over pts[0..B) (device): visited lists to
+// B_VIS_IDS / B_VIS_D with stride vcap, lengths to B_VISN, dist_cmps to
+// B_CMPS, the start point read from B_STARTS.  insert_search_prepare sizes
+// the buffers and picks the launch shape (it allocates, and may run the
+// filter pool's set-up, so it runs before any recorded launch);
+// insert_search_enqueue clears the work counter and the overflow word and
+// launches, with no host synchronisation (so it can be recorded).
+static ddg_status insert_search_prepare(ddg_index* idx, const uint32_t* pts, uint64_t B, uint32_t L, uint32_t vcap,
+                                        SearchArgs& a, Launch& l) {
+  ENSURE(B_VIS_IDS, (size_t)B * vcap * 4);
+  ENSURE(B_VIS_D, (size_t)B * vcap * 4);
+  ENSURE(B_VISN, (size_t)B * 4);
+  ENSURE(B_CMPS, (size_t)B * 8);
+  ENSURE(B_FLAGS, kFlagBytes);
+  ENSURE(B_STARTS, 64);
+  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+  a = SearchArgs{};
+  l = Launch{};
+  // DDG_BUILD_SHAPE="gtab,waves_per_block,blocks_per_cu,stage_rows" fixes
+  // the insert search's launch shape, for measurement (results are the
+  // same for every shape)
+  ddg_index::Tuned bcfg{};
+  const char* bshape = getenv("DDG_BUILD_SHAPE");
+  const bool pinned = bshape && sscanf(bshape, "%d,%u,%u,%u", &bcfg.gtab, &bcfg.wpb, &bcfg.bpc, &bcfg.stage) == 4;
+  if (pinned ? !search_layout_cfg(idx, L, 1, DDG_TRAV_NATIVE, &bcfg, a, l)
+             : !search_layout(idx, L, 1, DDG_TRAV_NATIVE, a, l, false))
+    return fail(idx, DDG_EINVAL, "build beam L too large for LDS");
+  a.base = idx->rows;
+  a.graph = idx->graph;
+  a.qids = pts;
+  a.starts = buf<uint32_t>(idx, B_STARTS);
+  a.dist_cmps = buf<uint64_t>(idx, B_CMPS);
+  a.visited_count = buf<uint32_t>(idx, B_VISN);
+  a.vis_ids = buf<uint32_t>(idx, B_VIS_IDS);
+  a.vis_dists = buf<float>(idx, B_VIS_D);
+  a.overflow = flags + F_OVERFLOW;
+  a.work = flags + F_WORK;
+  a.n = idx->n;
+  a.nq = B;
+  a.limit = idx->n;
+  a.cut = 0.0;
+  a.d = idx->d;
+  a.ad = idx->ad;
+  a.R = idx->R;
+  a.nstarts = 1;
+  a.k = 0;
+  a.beam = L;
+  a.degree_limit = idx->R;
+  a.vcap = vcap;
+  a.build_mode = 1;
+  a.cut_trim = 0;
+  l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (B + l.waves_per_block - 1) / l.waves_per_block);
+  ENSURE_GTABLE(a, l);
+  return DDG_OK;
+}
+
+static hipError_t insert_search_enqueue(ddg_index* idx, const SearchArgs& a, const Launch& l) {
+  hipError_t e = launch_fill_u32(buf<uint32_t>(idx, B_FLAGS) + F_WORK, 0u, 2, idx->stream);
+  if (e != hipSuccess) return e;
+  return launch_search(idx->dtype, idx->metric, DDG_TRAV_NATIVE, a, l, idx->stream);
+}
+
+// The build-mode search on its own (ddg_insert_search): a visited list that
+// outgrows its slot reruns the search with room to spare (*vcap doubled).
 static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_t B, uint32_t L, uint32_t start,
-                                    uint32_t* vcap, uint64_t* cmps_total) {
+                                    uint32_t* vcap) {
   for (int attempt = 0; attempt < 8; attempt++) {
-    ENSURE(B_VIS_IDS, (size_t)B * *vcap * 4);
-    ENSURE(B_VIS_D, (size_t)B * *vcap * 4);
-    ENSURE(B_VISN, (size_t)B * 4);
-    ENSURE(B_CMPS, (size_t)B * 8);
-    ENSURE(B_FLAGS, kFlagBytes);
-    ENSURE(B_STARTS, 64);
-    HIPCHK(hipMemcpyAsync(buf<uint32_t>(idx, B_STARTS), &start, 4, hipMemcpyHostToDevice, idx->stream));
-    uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
-    HIPCHK(launch_fill_u32(flags + F_WORK, 0u, 2, idx->stream));
     SearchArgs a{};
     Launch l{};
-    // DDG_BUILD_SHAPE="gtab,waves_per_block,blocks_per_cu,stage_rows" fixes
-    // the insert search's launch shape, for measurement (results are the
-    // same for every shape)
-    ddg_index::Tuned bcfg{};
-    const char* bshape = getenv("DDG_BUILD_SHAPE");
-    const bool pinned = bshape && sscanf(bshape, "%d,%u,%u,%u", &bcfg.gtab, &bcfg.wpb, &bcfg.bpc, &bcfg.stage) == 4;
-    if (pinned ? !search_layout_cfg(idx, L, 1, DDG_TRAV_NATIVE, &bcfg, a, l)
-               : !search_layout(idx, L, 1, DDG_TRAV_NATIVE, a, l, false))
-      return fail(idx, DDG_EINVAL, "build beam L too large for LDS");
-    a.base = idx->rows;
-    a.graph = idx->graph;
-    a.qids = pts;
-    a.starts = buf<uint32_t>(idx, B_STARTS);
-    a.dist_cmps = buf<uint64_t>(idx, B_CMPS);
-    a.visited_count = buf<uint32_t>(idx, B_VISN);
-    a.vis_ids = buf<uint32_t>(idx, B_VIS_IDS);
-    a.vis_dists = buf<float>(idx, B_VIS_D);
-    a.overflow = flags + 1;
-    a.work = flags;
-    a.n = idx->n;
-    a.nq = B;
-    a.limit = idx->n;
-    a.cut = 0.0;
-    a.d = idx->d;
-    a.ad = idx->ad;
-    a.R = idx->R;
-    a.nstarts = 1;
-    a.k = 0;
-    a.beam = L;
-    a.degree_limit = idx->R;
-    a.vcap = *vcap;
-    a.build_mode = 1;
-    a.cut_trim = 0;
-    l.blocks = (uint32_t)std::min<uint64_t>(l.blocks, (B + l.waves_per_block - 1) / l.waves_per_block);
-    ENSURE_GTABLE(a, l);
+    ddg_status st = insert_search_prepare(idx, pts, B, L, *vcap, a, l);
+    if (st != DDG_OK) return st;
+    HIPCHK(hipMemcpyAsync(buf<uint32_t>(idx, B_STARTS), &start, 4, hipMemcpyHostToDevice, idx->stream));
     Timer t(idx, &idx->prof[0]);
-    HIPCHK(launch_search(idx->dtype, idx->metric, DDG_TRAV_NATIVE, a, l, idx->stream));
+    HIPCHK(insert_search_enqueue(idx, a, l));
     t.stop();
     uint32_t ov = 0;
-    HIPCHK(hipMemcpy(&ov, flags + 1, 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&ov, buf<uint32_t>(idx, B_FLAGS) + F_OVERFLOW, 4, hipMemcpyDeviceToHost));
     if (ov & 2u) return fail(idx, DDG_ERANGE, "start point out of range");
-    if (!ov) {
-      if (cmps_total) {
-        std::vector<uint64_t> c(B);
-        HIPCHK(hipMemcpy(c.data(), buf<uint64_t>(idx, B_CMPS), B * 8, hipMemcpyDeviceToHost));
-        for (uint64_t x : c) *cmps_total += x;
-      }
-      return DDG_OK;
-    }
-    *vcap *= 2;  // a visited list outgrew its slot: rerun the batch with room to spare
+    if (!ov) return DDG_OK;
+    *vcap *= 2;
   }
   return fail(idx, DDG_EINVAL, "visited lists exceed the capacity limit");
 }
@@ -633,9 +703,28 @@ static size_t max_batch_size(double max_fraction, uint64_t count) {
   return mb == 0 ? (size_t)count : mb;
 }
 
-__global__ void k_acc_visited(uint32_t* vc, const uint32_t* pts, const uint32_t* visn, uint64_t B) {
+__global__ void k_acc_visited(uint32_t* vc, const uint32_t* pts, const uint32_t* visn, const uint64_t* B_dev) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < B) vc[pts[i]] += visn[i];
+  if (i < *B_dev) vc[pts[i]] += visn[i];
+}
+
+// A build round's size as the kernels after its insert search see it: B, or
+// 0 when a visited list outgrew its slot (or the start point was out of
+// range), so that the rest of the round changes nothing and the host can
+// rerun it with room to spare.
+__global__ void k_round_gate(uint32_t* flags, uint64_t B) {
+  if (threadIdx.x == 0) *(uint64_t*)(flags + F_ROUNDB) = flags[F_OVERFLOW] ? 0ull : B;
+}
+
+// dist_cmps of a round's insert searches added to the build's total.
+__global__ void k_sum_cmps(const uint64_t* cmps, const uint64_t* B_dev, unsigned long long* total) {
+  __shared__ unsigned long long part[4];
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = i < *B_dev ? (unsigned long long)cmps[i] : 0ull;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
 }
 
 __global__ void k_max_group(const uint64_t* goff, const uint64_t* ngroups, uint32_t* out) {
@@ -683,6 +772,7 @@ static ddg_status round_prepare(ddg_index* idx, const RoundShape& r, PrunePlan& 
   fwd.a.alpha = r.alpha;
   fwd.a.Rsel = r.Rsel;
   fwd.a.add = 1;
+  fwd.a.np_dev = (const uint64_t*)(buf<uint32_t>(idx, B_FLAGS) + F_ROUNDB);
   ddg_status st = prune_prepare(idx, fwd, B, r.vcap + idx->R, B * (uint64_t)r.vcap, kForwardCcap, true);
   if (st != DDG_OK) return st;
   // reverse append / prune per target (index.h:309-322); the cap keeps at
@@ -700,15 +790,33 @@ static ddg_status round_prepare(ddg_index* idx, const RoundShape& r, PrunePlan& 
   return prune_prepare(idx, rev, Pmax, rcap + 1 + idx->R, Pmax, rcap, false);
 }
 
-// Enqueues the round; no host synchronisation unless `timed` (then each part
-// is timed into the profile, which synchronises).
-static ddg_status round_enqueue(ddg_index* idx, const RoundShape& r, const PrunePlan& fwd0, const PrunePlan& rev0,
-                                bool timed) {
+// Enqueues the whole round (index.h:270-323): the insert search, the gate
+// that turns the rest into no-ops when a visited list overflowed, the
+// visited-count and dist_cmps accumulation, then the tail.  No host
+// synchronisation unless `timed` (then each part is timed into the profile,
+// which synchronises).
+static ddg_status round_enqueue(ddg_index* idx, const RoundShape& r, const SearchArgs& sa, const Launch& sl,
+                                uint32_t* vc_dev, const PrunePlan& fwd0, const PrunePlan& rev0, bool timed) {
   const uint64_t B = r.B, Pmax = r.B * r.Rsel;
   uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
   uint64_t* dpairs = (uint64_t*)(flags + F_PAIRS);
   uint64_t* dgroups = (uint64_t*)(flags + F_GROUPS);
-  double t_fwd = 0.0, t_edges = 0.0, t_rev = 0.0;
+  const uint64_t* dB = (const uint64_t*)(flags + F_ROUNDB);
+  double t_search = 0.0, t_fwd = 0.0, t_edges = 0.0, t_rev = 0.0;
+  {
+    Timer t(idx, &t_search, timed);
+    HIPCHK(insert_search_enqueue(idx, sa, sl));
+    hipLaunchKernelGGL(k_round_gate, dim3(1), dim3(64), 0, idx->stream, flags, B);
+    HIPCHK(hipGetLastError());
+    const uint32_t nb = (uint32_t)((B + 255) / 256);
+    if (vc_dev)
+      hipLaunchKernelGGL(k_acc_visited, dim3(nb), dim3(256), 0, idx->stream, vc_dev, buf<uint32_t>(idx, B_RPTS),
+                         buf<uint32_t>(idx, B_VISN), dB);
+    hipLaunchKernelGGL(k_sum_cmps, dim3(nb), dim3(256), 0, idx->stream, buf<uint64_t>(idx, B_CMPS), dB,
+                       (unsigned long long*)(flags + F_CMPS));
+    HIPCHK(hipGetLastError());
+    t.stop();
+  }
   {
     Timer t(idx, &t_fwd, timed);
     HIPCHK(prune_enqueue(idx, fwd0));
@@ -720,12 +828,12 @@ static ddg_status round_enqueue(ddg_index* idx, const RoundShape& r, const Prune
   {
     Timer t(idx, &t_edges, timed);
     HIPCHK(launch_apply_rows(idx->graph, idx->R + 1, buf<uint32_t>(idx, B_RPTS), buf<uint32_t>(idx, B_NEW_IDS),
-                             buf<uint32_t>(idx, B_NEW_LEN), r.Rsel, B, flags + F_STATUS, idx->stream));
+                             buf<uint32_t>(idx, B_NEW_LEN), r.Rsel, B, flags + F_STATUS, idx->stream, dB));
     HIPCHK(scan_exclusive_u32(buf<uint32_t>(idx, B_NEW_LEN), buf<uint64_t>(idx, B_OFFS), B, dpairs,
-                              buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap, idx->stream));
+                              buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap, idx->stream, dB));
     HIPCHK(launch_emit_pairs(buf<uint32_t>(idx, B_RPTS), buf<uint32_t>(idx, B_NEW_IDS), buf<uint32_t>(idx, B_NEW_LEN),
                              buf<uint64_t>(idx, B_OFFS), r.Rsel, B, buf<uint32_t>(idx, B_KEYS),
-                             buf<uint32_t>(idx, B_VALS), idx->stream));
+                             buf<uint32_t>(idx, B_VALS), idx->stream, dB));
     bool alt = false;
     HIPCHK(radix_sort_pairs(buf<uint32_t>(idx, B_KEYS), buf<uint32_t>(idx, B_VALS), buf<uint32_t>(idx, B_KEYS2),
                             buf<uint32_t>(idx, B_VALS2), Pmax, r.kb, buf<void>(idx, B_TMP), idx->s.b[B_TMP].cap,
@@ -745,6 +853,7 @@ static ddg_status round_enqueue(ddg_index* idx, const RoundShape& r, const Prune
     t.stop();
   }
   if (timed) {
+    idx->prof[0] += t_search;
     idx->prof[1] += t_fwd;
     idx->prof[2] += t_edges;
     idx->prof[3] += t_rev;
@@ -752,11 +861,11 @@ static ddg_status round_enqueue(ddg_index* idx, const RoundShape& r, const Prune
   return DDG_OK;
 }
 
-// Round tails run as recorded hipGraphs (one per batch size B), unless
-// DDG_GRAPH=0 (then they run launch by launch with per-part timing).
+// Build rounds, insert search included, run as recorded hipGraphs (one per
+// batch size B), unless DDG_GRAPH=0 (then they run launch by launch with
+// per-part timing).
 static bool use_graphs() {
-  static const int g = env_int("DDG_GRAPH", 1, 0, 1);
-  return g == 1;
+  return env_int("DDG_GRAPH", 1, 0, 1) == 1;  // read at every build (tests switch it)
 }
 
 struct GraphCache {
@@ -774,77 +883,103 @@ static ddg_status batch_insert_dev(ddg_index* idx, const uint32_t* ins, uint64_t
                                    uint64_t max_batch_base, uint32_t* vc_dev) {
   std::vector<std::pair<uint64_t, uint64_t>> batches;
   schedule(m, base, max_batch_size(max_fraction, max_batch_base), batches);
+  // DDG_BUILD_VCAP=<v> starts the visited-list slots smaller (tests reach the
+  // overflow rerun with it; results are the same)
   uint32_t vcap = std::max<uint32_t>(2 * L + 64, 128);
+  const int vcap_env = env_int("DDG_BUILD_VCAP", 0, 1, 1 << 20);
+  if (vcap_env > 0) vcap = (uint32_t)vcap_env;
   const uint32_t kb = key_bits_for(idx->n);
-  uint64_t total_cmps = 0, Bmax = 0;
+  uint64_t Bmax = 0;
   for (auto [lo, hi] : batches) Bmax = std::max<uint64_t>(Bmax, hi - lo);
   if (Bmax == 0) return DDG_OK;
   // every buffer at its largest size up front: recorded rounds keep their pointers
   ENSURE(B_FLAGS, kFlagBytes);
   ENSURE(B_RPTS, Bmax * 4);
-  ENSURE(B_VIS_IDS, (size_t)Bmax * vcap * 4);
-  ENSURE(B_VIS_D, (size_t)Bmax * vcap * 4);
-  ENSURE(B_VISN, (size_t)Bmax * 4);
-  ENSURE(B_CMPS, (size_t)Bmax * 8);
-  {
-    PrunePlan f{}, r{};
-    ddg_status st = round_prepare(idx, RoundShape{Bmax, vcap, Rsel, kb, alpha}, f, r);
+  ENSURE(B_STARTS, 64);
+  auto prepare_max = [&]() -> ddg_status {
+    SearchArgs a{};
+    Launch l{};
+    ddg_status st = insert_search_prepare(idx, buf<uint32_t>(idx, B_RPTS), Bmax, L, vcap, a, l);
     if (st != DDG_OK) return st;
-  }
-  HIPCHK(launch_fill_u32(buf<uint32_t>(idx, B_FLAGS) + F_STATUS, 0u, 1, idx->stream));
+    PrunePlan f{}, r{};
+    return round_prepare(idx, RoundShape{Bmax, vcap, Rsel, kb, alpha}, f, r);
+  };
+  ddg_status pst = prepare_max();
+  if (pst != DDG_OK) return pst;
+  uint32_t* flags = buf<uint32_t>(idx, B_FLAGS);
+  HIPCHK(hipMemcpyAsync(buf<uint32_t>(idx, B_STARTS), &start, 4, hipMemcpyHostToDevice, idx->stream));
+  HIPCHK(launch_fill_u32(flags + F_STATUS, 0u, 1, idx->stream));
+  HIPCHK(launch_fill_u32(flags + F_CMPS, 0u, 2, idx->stream));
   const bool graphs = use_graphs() && !trace_on();
   GraphCache cache;
-  for (auto [lo, hi] : batches) {
-    const uint64_t Bn = hi - lo;
+  for (size_t bi = 0; bi < batches.size(); bi++) {
+    const uint64_t lo = batches[bi].first, Bn = batches[bi].second - lo;
     if (Bn == 0) continue;
-    const uint32_t* pts = ins + lo;
-    TRACE("batch [%llu, %llu): search", (unsigned long long)lo, (unsigned long long)hi);
-    const uint32_t vcap_before = vcap;
-    ddg_status st = run_insert_search(idx, pts, Bn, L, start, &vcap, &total_cmps);
-    if (st != DDG_OK) return st;
-    TRACE("  search done (vcap %u)", vcap);
-    if (vcap != vcap_before) cache.clear();  // visited buffers were reallocated
-    st = check_status(idx);  // the previous round's errors (the search synchronised)
-    if (st != DDG_OK) return st;
-    if (vc_dev)
-      hipLaunchKernelGGL(k_acc_visited, dim3((Bn + 255) / 256), dim3(256), 0, idx->stream, vc_dev, pts,
-                         buf<uint32_t>(idx, B_VISN), Bn);
-    HIPCHK(hipMemcpyAsync(buf<uint32_t>(idx, B_RPTS), pts, Bn * 4, hipMemcpyDeviceToDevice, idx->stream));
-    const RoundShape rs{Bn, vcap, Rsel, kb, alpha};
-    PrunePlan fwd{}, rev{};
-    st = round_prepare(idx, rs, fwd, rev);
-    if (st != DDG_OK) return st;
-    if (!graphs) {
-      st = round_enqueue(idx, rs, fwd, rev, true);
+    TRACE("batch [%llu, %llu)", (unsigned long long)lo, (unsigned long long)(lo + Bn));
+    HIPCHK(hipMemcpyAsync(buf<uint32_t>(idx, B_RPTS), ins + lo, Bn * 4, hipMemcpyDeviceToDevice, idx->stream));
+    // the whole round as one recorded graph per batch size (or launch by
+    // launch with DDG_GRAPH=0); a visited list that outgrew its slot leaves
+    // the graph unchanged, and the round is run again with slots twice as
+    // long
+    for (int attempt = 0;; attempt++) {
+      if (attempt == 8) return fail(idx, DDG_EINVAL, "visited lists exceed the capacity limit");
+      const RoundShape rs{Bn, vcap, Rsel, kb, alpha};
+      SearchArgs sa{};
+      Launch sl{};
+      PrunePlan fwd{}, rev{};
+      ddg_status st = insert_search_prepare(idx, buf<uint32_t>(idx, B_RPTS), Bn, L, vcap, sa, sl);
       if (st != DDG_OK) return st;
-      TRACE("  round tail done");
-      continue;
-    }
-    hipGraphExec_t exec = nullptr;
-    for (auto& e : cache.execs)
-      if (e.first == Bn) exec = e.second;
-    if (!exec) {
-      hipGraph_t g = nullptr;
-      HIPCHK(hipStreamBeginCapture(idx->stream, hipStreamCaptureModeThreadLocal));
-      st = round_enqueue(idx, rs, fwd, rev, false);
-      hipError_t ce = hipStreamEndCapture(idx->stream, &g);
-      if (st != DDG_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return st;
+      st = round_prepare(idx, rs, fwd, rev);
+      if (st != DDG_OK) return st;
+      if (!graphs) {
+        st = round_enqueue(idx, rs, sa, sl, vc_dev, fwd, rev, true);
+        if (st != DDG_OK) return st;
+      } else {
+        hipGraphExec_t exec = nullptr;
+        for (auto& e : cache.execs)
+          if (e.first == Bn) exec = e.second;
+        if (!exec) {
+          hipGraph_t g = nullptr;
+          HIPCHK(hipStreamBeginCapture(idx->stream, hipStreamCaptureModeThreadLocal));
+          st = round_enqueue(idx, rs, sa, sl, vc_dev, fwd, rev, false);
+          hipError_t ce = hipStreamEndCapture(idx->stream, &g);
+          if (st != DDG_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+          }
+          if (ce != hipSuccess) return fail(idx, DDG_EDEVICE, std::string("round capture: ") + hipGetErrorString(ce));
+          ce = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(g);
+          if (ce != hipSuccess)
+            return fail(idx, DDG_EDEVICE, std::string("round instantiate: ") + hipGetErrorString(ce));
+          cache.execs.push_back({Bn, exec});
+        }
+        Timer t(idx, &idx->prof[2]);
+        HIPCHK(hipGraphLaunch(exec, idx->stream));
+        t.stop();
       }
-      if (ce != hipSuccess) return fail(idx, DDG_EDEVICE, std::string("round capture: ") + hipGetErrorString(ce));
-      ce = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (ce != hipSuccess) return fail(idx, DDG_EDEVICE, std::string("round instantiate: ") + hipGetErrorString(ce));
-      cache.execs.push_back({Bn, exec});
+      // the round's overflow and error words (this read waits for the round)
+      uint32_t w[8];
+      HIPCHK(hipMemcpy(w, flags, sizeof(w), hipMemcpyDeviceToHost));
+      if (w[F_OVERFLOW] & 2u) return fail(idx, DDG_ERANGE, "start point out of range");
+      if (w[F_STATUS] & 1u) return fail(idx, DDG_EDEGREE, "adjacency list would exceed max degree");
+      if (w[F_STATUS] & 2u) return fail(idx, DDG_EINVAL, "prune candidate list exceeds scratch capacity");
+      if (!(w[F_OVERFLOW] & 1u)) break;
+      // every buffer again at its largest size for the longer slots; the
+      // recorded rounds point at the old ones
+      vcap *= 2;
+      cache.clear();
+      st = prepare_max();
+      if (st != DDG_OK) return st;
+      TRACE("  visited lists overflowed: rerunning the round with vcap %u", vcap);
     }
-    Timer t(idx, &idx->prof[2]);
-    HIPCHK(hipGraphLaunch(exec, idx->stream));
-    t.stop();
+    TRACE("  round done (vcap %u)", vcap);
   }
   HIPCHK(hipStreamSynchronize(idx->stream));
   ddg_status st = check_status(idx);
   if (st != DDG_OK) return st;
+  uint64_t total_cmps = 0;
+  HIPCHK(hipMemcpy(&total_cmps, flags + F_CMPS, 8, hipMemcpyDeviceToHost));
   idx->prof[7] += (double)total_cmps;
   return DDG_OK;
 }
@@ -1755,7 +1890,7 @@ ddg_status ddg_insert_search(ddg_index* idx, const uint32_t* ids, uint64_t m, ui
   ENSURE(B_INSERTS, m * 4);
   HIPCHK(hipMemcpy(buf<uint32_t>(idx, B_INSERTS), ids, m * 4, hipMemcpyHostToDevice));
   uint32_t cap = vcap;
-  ddg_status st = run_insert_search(idx, buf<uint32_t>(idx, B_INSERTS), m, L, start_point, &cap, nullptr);
+  ddg_status st = run_insert_search(idx, buf<uint32_t>(idx, B_INSERTS), m, L, start_point, &cap);
   if (st != DDG_OK) return st;
   std::vector<uint32_t> vi((size_t)m * cap);
   std::vector<float> vd((size_t)m * cap);

@@ -17,6 +17,15 @@ constexpr uint32_t kTile = kBlock * kItems;  // 4096 elements per block
 
 static inline uint32_t blocks_for(uint64_t n, uint64_t per) { return (uint32_t)((n + per - 1) / per); }
 
+// Sizes given as (n, n_dev): n bounds the grid, n_dev (nullable) holds the
+// count the kernel honours, so a launch can be recorded before its input
+// size is known (graph capture of a build round).
+__device__ __forceinline__ uint64_t dev_count(uint64_t n, const uint64_t* n_dev) {
+  if (!n_dev) return n;
+  const uint64_t v = *n_dev;
+  return v < n ? v : n;
+}
+
 // ------------------------------------------------------------- small helpers
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,9 +77,10 @@ hipError_t launch_pad_rows(void* rows, uint64_t n, uint32_t d_bytes, uint32_t ro
 
 // --------------------------------------------------------- apply forward rows
 __global__ void k_apply_rows(uint32_t* graph, uint32_t R1, const uint32_t* pts, const uint32_t* new_ids,
-                             const uint32_t* new_len, uint32_t Rsel, uint64_t B, uint32_t* status) {
+                             const uint32_t* new_len, uint32_t Rsel, uint64_t B_max, const uint64_t* B_dev,
+                             uint32_t* status) {
   const uint64_t b = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (b >= B) return;
+  if (b >= dev_count(B_max, B_dev)) return;
   const uint32_t lane = lane_id();
   const uint32_t len = new_len[b];
   if (len > R1 - 1) {
@@ -83,10 +93,11 @@ __global__ void k_apply_rows(uint32_t* graph, uint32_t R1, const uint32_t* pts, 
 }
 
 hipError_t launch_apply_rows(uint32_t* graph, uint32_t R1, const uint32_t* pts, const uint32_t* new_ids,
-                             const uint32_t* new_len, uint32_t Rsel, uint64_t B, uint32_t* status, hipStream_t s) {
+                             const uint32_t* new_len, uint32_t Rsel, uint64_t B, uint32_t* status, hipStream_t s,
+                             const uint64_t* B_dev) {
   if (B == 0) return hipSuccess;
   hipLaunchKernelGGL(k_apply_rows, dim3(blocks_for(B, 4)), dim3(256), 0, s, graph, R1, pts, new_ids, new_len, Rsel,
-                     B, status);
+                     B, B_dev, status);
   return hipGetLastError();
 }
 
@@ -114,14 +125,6 @@ __device__ __forceinline__ uint64_t block_scan_excl(uint64_t v, uint64_t* sh, ui
   return woff + x - v;
 }
 
-// Sizes given as (n, n_dev): n bounds the grid, n_dev (nullable) holds the
-// count the kernel honours, so a launch can be recorded before its input
-// size is known (graph capture of a build round).
-__device__ __forceinline__ uint64_t dev_count(uint64_t n, const uint64_t* n_dev) {
-  if (!n_dev) return n;
-  const uint64_t v = *n_dev;
-  return v < n ? v : n;
-}
 
 __global__ void k_scan_reduce(const uint32_t* in, uint64_t n_max, const uint64_t* n_dev, uint64_t* bsum) {
   __shared__ uint64_t sh[4];
@@ -189,9 +192,10 @@ hipError_t scan_exclusive_u32(const uint32_t* in, uint64_t* out, uint64_t n, uin
 
 // --------------------------------------------------------------- emit pairs
 __global__ void k_emit_pairs(const uint32_t* pts, const uint32_t* new_ids, const uint32_t* new_len,
-                             const uint64_t* offs, uint32_t Rsel, uint64_t B, uint32_t* keys, uint32_t* vals) {
+                             const uint64_t* offs, uint32_t Rsel, uint64_t B_max, const uint64_t* B_dev, uint32_t* keys,
+                             uint32_t* vals) {
   const uint64_t b = (uint64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (b >= B) return;
+  if (b >= dev_count(B_max, B_dev)) return;
   const uint32_t lane = lane_id();
   const uint32_t len = new_len[b];
   const uint64_t o = offs[b];
@@ -204,10 +208,10 @@ __global__ void k_emit_pairs(const uint32_t* pts, const uint32_t* new_ids, const
 
 hipError_t launch_emit_pairs(const uint32_t* pts, const uint32_t* new_ids, const uint32_t* new_len,
                              const uint64_t* offs, uint32_t Rsel, uint64_t B, uint32_t* keys, uint32_t* vals,
-                             hipStream_t s) {
+                             hipStream_t s, const uint64_t* B_dev) {
   if (B == 0) return hipSuccess;
   hipLaunchKernelGGL(k_emit_pairs, dim3(blocks_for(B, 4)), dim3(256), 0, s, pts, new_ids, new_len, offs, Rsel, B,
-                     keys, vals);
+                     B_dev, keys, vals);
   return hipGetLastError();
 }
 

@@ -50,6 +50,10 @@ struct SearchArgs {
   // resident wave, indexed by block * waves_per_block + wave
   uint32_t* gtable;
   uint32_t gtable_words;
+  // epoch-tagged slices (null: each query clears its slice): one epoch word
+  // per slice; stored words are id | epoch << id_bits, epoch < epoch_end
+  uint32_t* gepoch;
+  uint32_t id_bits, epoch_end;
   // rows longer than one load batch are gathered into LDS cooperatively:
   // stage_rows rows of stage_chunks 16-byte chunks, stage_stride apart
   // (chunks), at off_stage; stage_rows = 0 keeps one lane per row
@@ -122,8 +126,10 @@ constexpr uint32_t kMaxBigWaves = 16;
 uint32_t prune_big_lds_bytes(uint32_t row_bytes, uint32_t Rsel);
 
 // edges.hip
+// B_dev (nullable): the batch size the kernels honour, min(*B_dev, B)
 hipError_t launch_apply_rows(uint32_t* graph, uint32_t R1, const uint32_t* pts, const uint32_t* new_ids,
-                             const uint32_t* new_len, uint32_t Rsel, uint64_t B, uint32_t* status, hipStream_t s);
+                             const uint32_t* new_len, uint32_t Rsel, uint64_t B, uint32_t* status, hipStream_t s,
+                             const uint64_t* B_dev = nullptr);
 // Launchers taking (n, n_dev): n sizes the grid; when n_dev is not null the
 // kernels process min(*n_dev, n) elements (the count is produced on the
 // device by an earlier kernel of the same round).
@@ -132,7 +138,7 @@ hipError_t scan_exclusive_u32(const uint32_t* in, uint64_t* out, uint64_t n, uin
 size_t scan_tmp_bytes(uint64_t n);
 hipError_t launch_emit_pairs(const uint32_t* pts, const uint32_t* new_ids, const uint32_t* new_len,
                              const uint64_t* offs, uint32_t Rsel, uint64_t B, uint32_t* keys, uint32_t* vals,
-                             hipStream_t s);
+                             hipStream_t s, const uint64_t* B_dev = nullptr);
 size_t radix_tmp_bytes(uint64_t n);
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, uint32_t* keys_alt, uint32_t* vals_alt, uint64_t n,
                             uint32_t key_bits, void* tmp, size_t tmp_bytes, hipStream_t s, bool* result_in_alt,

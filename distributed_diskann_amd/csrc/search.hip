@@ -133,6 +133,15 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 
   // the bitmap is used by the native kernels only (the host sets wbits
   // only for them); the reduced-precision kernels compile without it
   const bool WB = GTAB && TRAV == TRAV_NATIVE && a.wbits != 0u;
+  // Epoch-tagged global filter (no bitmap, a.gepoch set): the slice stores
+  // id | epoch << id_bits, with a per-slice epoch advanced once per query,
+  // so a query starts without clearing its slice: a word an earlier query
+  // stored carries another epoch and equals no tagged id of this one, as an
+  // empty slot would.  The slice is cleared only when the epoch wraps, and
+  // its epoch is kept in a.gepoch across launches.
+  const bool EP = GTAB && !WB && a.gepoch != nullptr;
+  const uint32_t slice_ix = blockIdx.x * (blockDim.x >> 6) + wv;
+  uint32_t epoch = EP ? bcast0(a.gepoch[slice_ix]) : 0u;
 
   const uint64_t R1 = (uint64_t)a.R + 1;
   const T* base = (const T*)a.base;
@@ -167,16 +176,20 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 
       wave_sync_global();
       if (WB) {
         for (uint32_t i = lane; i < (mask + 1u) / 32u; i += kWave) wb[i] = 0u;
+      } else if (EP && epoch + 1u < a.epoch_end) {
+        epoch++;
       } else {
         for (uint32_t i = lane; i < (mask + 1u) / 4u; i += kWave)
           ((uint4*)table)[i] = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
         wave_sync_global();
+        epoch = 0;
       }
     } else {
       for (uint32_t i = lane; i <= mask; i += kWave) table[i] = 0xFFFFFFFFu;
     }
     wave_sync();
     const typename P::Q qs = P::prep(a, ql);
+    const uint32_t etag = EP ? epoch << a.id_bits : 0u;
 
     // starting frontier, sorted by (distance, id).  A start id outside
     // [0, n) (device memory the host could not check) raises bit 1 of the
@@ -290,6 +303,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 
         const bool valid = j < m;
         const uint32_t av = valid ? (b0 == 0 ? nb0 : row[1 + j]) : 0u;
         const bool act = valid && (av != self_id);
+        const uint32_t tav = av | etag;  // the word this neighbor stores (tagged with the epoch)
         const bool pre = b0 == 0 && use_pf_table;  // slot and table word loaded ahead
         const uint32_t slot = pre ? pf_slot : (uint32_t)hash64_2((uint64_t)av) & mask;
         // table stores of the previous 64 neighbors by other lanes (those of
@@ -324,17 +338,17 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 
             const int jj = first_lane(it);
             it &= it - 1;
             const uint32_t sj = readlane_u32(slot, jj);
-            const uint32_t aj = readlane_u32(av, jj);
+            const uint32_t aj = readlane_u32(tav, jj);
             if (act && sj == slot) {
               if ((uint32_t)jj < lane) prev = aj;
               else if ((uint32_t)jj > lane) has_next = true;
             }
           }
         }
-        const bool keep = act && (prev != av);
+        const bool keep = act && (prev != tav);
         wave_sync();
-        if (act && !has_next && tv != av) {  // the last writer of a slot; skip same-value stores
-          table[slot] = av;
+        if (act && !has_next && tv != tav) {  // the last writer of a slot; skip same-value stores
+          table[slot] = tav;
           if (WB) atomicOr(&wb[slot >> 5], 1u << (slot & 31u));
         }
         const uint64_t km = ballot(keep);
@@ -676,6 +690,7 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 
     }
     wave_sync();
   }
+  if (EP && lane_id() == 0) a.gepoch[slice_ix] = epoch;
 }
 
 // Distances of query q (wave q) to ids[q*m .. q*m+m), computed exactly as the

@@ -198,3 +198,18 @@ def test_reverse_pair_grouping(n_pairs, n_keys):
     g = int(ng[0])
     assert g == len(uk) and np.array_equal(gk[:g], uk) and np.array_equal(go[:g], starts)
     assert go[g] == n_pairs and mg[0] == counts.max()
+
+
+@pytest.mark.parametrize("vcap,graphs", [("8", "1"), ("24", "1"), ("8", "0")])
+def test_build_visited_overflow_rerun(oracle, monkeypatch, vcap, graphs):
+    """Visited-list slots far shorter than the insert searches' lists: every
+    early round overflows inside its recorded graph (whose later kernels then
+    change nothing), is rerun with slots twice as long, and the build still
+    equals the oracle's row for row; the same launch by launch."""
+    print(f"[gpu] build with visited slots starting at {vcap} (graphs {graphs})", flush=True)
+    monkeypatch.setenv("DDG_BUILD_VCAP", vcap)
+    monkeypatch.setenv("DDG_GRAPH", graphs)
+    Xa, Qa, P, _ = _small_case(oracle, "float32", "l2", 6000, 24, seed=5, R=24, L=48)
+    G = oracle.build(P, 24, 48, 1.2, tie_mode=oracle.TIE_ID)
+    idx, rows, secs = _gpu_build(Xa, 24, "l2", 24, L=48, alpha=1.2)
+    assert np.array_equal(canon(rows), canon(G.rows))
