@@ -167,6 +167,23 @@ static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
   if (trav == DDG_TRAV_NATIVE) {
     a.qrow_bytes = round_up(idx->ad * idx->es, 16);
     a.off_qaux = a.qrow_bytes;
+    // float32 rows of more than one load batch whose reference order walks
+    // whole groups of four (MIPS, or L2 with d % 4 == 0) are split over the
+    // fewest lanes (a power of two) that leave each at most kSplitChunks
+    // chunks: one round trip per row instead of one per batch of eight
+    // chunks.  DDG_NAT_SPLIT=0 keeps one lane per row (for measurement; the
+    // results are the same).
+    a.nat_split = 0;
+    a.nat_cps = 0;
+    const uint32_t chunks = idx->ad / 4;
+    const char* ns = getenv("DDG_NAT_SPLIT");
+    if (idx->dtype == DDG_F32 && chunks > kChunkBatch && (idx->metric == DDG_MIPS || idx->d % 4 == 0) &&
+        !(ns && ns[0] == '0')) {
+      uint32_t S = 2;
+      while (S < kWave && (chunks + S - 1) / S > kSplitChunks) S *= 2;
+      a.nat_split = S;
+      a.nat_cps = (chunks + S - 1) / S;
+    }
     return a.qrow_bytes;
   }
   const uint32_t esz = trav_esize(idx, trav);

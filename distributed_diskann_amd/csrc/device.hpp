@@ -292,6 +292,81 @@ struct Dist<float, METRIC> {
   }
 };
 
+// Dist<float, METRIC>::dist of one row split over a group of S lanes (S a
+// power of two, groups aligned in the wave), for rows that walk whole
+// groups of four (MIPS, or L2 with d % 4 == 0) in ng chunks.  Lane `sub` of
+// the group loads chunks [sub * cps, sub * cps + cps) together (one round
+// trip for the whole row instead of one per batch of eight), forms its
+// products (MIPS) or per-group sums of squares (L2) with the same roundings
+// as the one-lane walk, and then the running sum is handed along the group:
+// lane sub adds its terms, in order, to the sum lane sub - 1 passes on.  The
+// adds are the one-lane walk's, in its order, so the bits are the same.  All
+// lanes of the wave call it with the same S / cps; every lane's row pointer
+// must be valid (lanes without a row pass any valid row, their result is
+// discarded).  The last lane of each group returns the distance.
+constexpr uint32_t kSplitChunks = 8;  // at most this many chunks per lane
+
+template <int METRIC>
+__device__ __forceinline__ float split_row_dist(const float* row, const float* q, uint32_t ng, uint32_t S,
+                                                uint32_t cps, uint32_t sub) {
+  const float4* q4 = (const float4*)q;
+  const uint4* r = (const uint4*)row;
+  const uint32_t cb = sub * cps;
+  uint4 xs[kSplitChunks];
+#pragma unroll
+  for (uint32_t u = 0; u < kSplitChunks; u++) {
+    const uint32_t c = cb + u;
+    xs[u] = r[(u < cps && c < ng) ? c : 0u];
+  }
+  // the terms this lane adds: four products per chunk (MIPS), or one sum of
+  // squares per chunk (L2: ((e0^2 + e1^2) + e2^2) + e3^2, NSGDist.h:113-131)
+  float t[kSplitChunks][4];
+  bool ok[kSplitChunks];
+#pragma unroll
+  for (uint32_t u = 0; u < kSplitChunks; u++) {
+    const uint32_t c = cb + u;
+    ok[u] = u < cps && c < ng;
+    const float4 b = q4[ok[u] ? c : 0u];
+    const float4 a = make_float4(__uint_as_float(xs[u].x), __uint_as_float(xs[u].y), __uint_as_float(xs[u].z),
+                                 __uint_as_float(xs[u].w));
+    if (METRIC == L2) {
+      const float e0 = a.x - b.x, e1 = a.y - b.y, e2 = a.z - b.z, e3 = a.w - b.w;
+      float gs = e0 * e0;
+      gs = gs + e1 * e1;
+      gs = gs + e2 * e2;
+      gs = gs + e3 * e3;
+      t[u][0] = gs;
+    } else {
+      t[u][0] = b.x * a.x;
+      t[u][1] = b.y * a.y;
+      t[u][2] = b.z * a.z;
+      t[u][3] = b.w * a.w;
+    }
+  }
+  float acc = 0.f;
+  for (uint32_t s = 0; s < S; s++) {
+    const float in = __shfl_up(acc, 1);
+    if (sub == s) {
+      float v = s == 0 ? 0.f : in;
+#pragma unroll
+      for (uint32_t u = 0; u < kSplitChunks; u++) {
+        float w = v;
+        if (METRIC == L2) {
+          w = w + t[u][0];
+        } else {
+          w = w + t[u][0];
+          w = w + t[u][1];
+          w = w + t[u][2];
+          w = w + t[u][3];
+        }
+        v = ok[u] ? w : v;
+      }
+      acc = v;
+    }
+  }
+  return METRIC == L2 ? acc : -acc;
+}
+
 // Copies one aligned row (bytes a multiple of 16) into LDS with the wave.
 __device__ __forceinline__ void stage_row(void* dst, const void* src, uint32_t bytes) {
   uint4* o = (uint4*)dst;

@@ -46,6 +46,9 @@ struct SearchArgs {
   const float* trav_scale;
   uint32_t trav_stride, trav_chunks, rerank;
   uint32_t trav_split, trav_cps;  // lanes per reduced row and chunks per lane (split_dist)
+  // native float32 rows of more than kChunkBatch chunks: lanes per row (0 or 1:
+  // one lane per row) and chunks per lane (split_row_dist)
+  uint32_t nat_split, nat_cps;
   // visited filter in global memory (null: in LDS), gtable_words per
   // resident wave, indexed by block * waves_per_block + wave
   uint32_t* gtable;

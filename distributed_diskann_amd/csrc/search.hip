@@ -28,6 +28,8 @@
 //    frontier can never re-enter it (its distance is >= the cutoff, or above
 //    cut * d_k once trimmed, and both only shrink), so the flag is exact for
 //    cut >= 1 (the host rejects cut < 1 with k > 0).
+#include <type_traits>
+
 #include "device.hpp"
 #include "internal.hpp"
 #include "trav.hpp"
@@ -35,6 +37,7 @@
 namespace ddg {
 
 constexpr uint32_t kBuckets = 1024;  // slot buckets for the lane-conflict check
+constexpr int kRowsLane = 0, kRowsStage = 1, kRowsSplit = 2;
 
 // Frontier and candidate entries are (distance bits, id) pairs; bit 31 of a
 // frontier id marks a visited entry.  Order: (distance, id) of
@@ -110,10 +113,16 @@ __device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t 
 // uint8 at Ls=140, 3.37 ms at 5 against 3.60 ms at 6; the 10M builds 9-14 %
 // faster.  The BF16 kernels take 5 (96 VGPRs; 1.39 -> 1.23 ms on 100K x 20,
 // the same as 4 at 10M); FP8 keeps the compiler's choice (no gain at 5).
-template <typename T, int METRIC, int TRAV, bool GTAB, bool STAGE>
-__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 ? 5 : 1))) void k_search(SearchArgs a) {
+// ROWS: how candidate rows reach their distances: kRowsLane (one lane per
+// row), kRowsStage (gathered into LDS first) or kRowsSplit (native float32
+// rows split over a.nat_split lanes, split_row_dist); each is its own
+// instantiation, so no kernel carries another's registers.
+template <typename T, int METRIC, int TRAV, bool GTAB, int ROWS>
+__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? (ROWS == kRowsSplit ? 4 : 5) : (TRAV == TRAV_BF16 ? 5 : 1)))
+void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using P = TravDist<T, METRIC, TRAV>;
+  constexpr bool STAGE = ROWS == kRowsStage;
   const uint32_t lane = lane_id();
   // the wave's index in its block, as a scalar: every per-wave pointer below
   // is then wave-uniform (SGPRs), not per-lane
@@ -435,6 +444,26 @@ __global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? 5 : (TRAV == TRAV_BF16 
         // pending load at a loop header, which would stall on the prefetch
         if (nk > 0) batch(0);
         for (uint32_t b0 = step; b0 < nk; b0 += step) batch(b0);
+      } else if constexpr (ROWS == kRowsSplit) {
+        // long float32 rows: one row over nat_split lanes, all its chunks in
+        // flight at once, the running sum handed along the lanes in the
+        // reference's order (split_row_dist)
+        const uint32_t S = a.nat_split;
+        const uint32_t sub = lane & (S - 1u);
+        const uint32_t step = kWave / S;
+        auto batch = [&](uint32_t b0) {
+          const uint32_t j = b0 + lane / S;
+          const bool v = j < nk;
+          const uint32_t id = Kid[v ? j : b0];  // lanes past the list read a row of the batch
+          const float dd = split_row_dist<METRIC>((const float*)a.base + (uint64_t)id * a.ad, (const float*)ql,
+                                                  a.ad >> 2, S, a.nat_cps, sub);
+          const bool pass = v && sub == S - 1u && !(dd >= cutoff);
+          const uint64_t pm = ballot(pass);
+          if (pass) Cn[nc + mbcnt(pm)] = cand_key(dd, id);
+          nc += popc(pm);
+        };
+        if (nk > 0) batch(0);  // outside the loop, as above
+        for (uint32_t b0 = step; b0 < nk; b0 += step) batch(b0);
       } else {
         auto batch = [&](uint32_t b0) {
           const uint32_t j = b0 + lane;
@@ -712,18 +741,26 @@ __global__ __launch_bounds__(256) void k_trav_dist(SearchArgs a, const uint32_t*
   }
 }
 
-template <typename T, int M, int TR, bool G, bool S>
+template <typename T, int M, int TR, bool G, int RW>
 static hipError_t launch_g(const SearchArgs& a, const Launch& l, hipStream_t s) {
-  hipLaunchKernelGGL((k_search<T, M, TR, G, S>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s,
+  hipLaunchKernelGGL((k_search<T, M, TR, G, RW>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s,
                      a);
   return hipGetLastError();
 }
 
+// split rows exist for native float32 only
+template <typename T, int TR>
+constexpr bool has_split() { return std::is_same<T, float>::value && TR == TRAV_NATIVE; }
+
 template <typename T, int M, int TR>
 static hipError_t launch_t(const SearchArgs& a, const Launch& l, hipStream_t s) {
   if (a.stage_rows)
-    return a.gtable ? launch_g<T, M, TR, true, true>(a, l, s) : launch_g<T, M, TR, false, true>(a, l, s);
-  return a.gtable ? launch_g<T, M, TR, true, false>(a, l, s) : launch_g<T, M, TR, false, false>(a, l, s);
+    return a.gtable ? launch_g<T, M, TR, true, kRowsStage>(a, l, s) : launch_g<T, M, TR, false, kRowsStage>(a, l, s);
+  if constexpr (has_split<T, TR>()) {
+    if (a.nat_split > 1u)
+      return a.gtable ? launch_g<T, M, TR, true, kRowsSplit>(a, l, s) : launch_g<T, M, TR, false, kRowsSplit>(a, l, s);
+  }
+  return a.gtable ? launch_g<T, M, TR, true, kRowsLane>(a, l, s) : launch_g<T, M, TR, false, kRowsLane>(a, l, s);
 }
 
 template <typename T, int M, int TR>
@@ -737,10 +774,14 @@ static hipError_t launch_d(const SearchArgs& a, const uint32_t* ids, uint32_t m,
 template <typename T, int M, int TR>
 static void set_attrs_t(int lds) {
   const hipFuncAttribute at = hipFuncAttributeMaxDynamicSharedMemorySize;
-  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, false>, at, lds);
-  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, false>, at, lds);
-  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, true>, at, lds);
-  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, true>, at, lds);
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, kRowsLane>, at, lds);
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, kRowsLane>, at, lds);
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, kRowsStage>, at, lds);
+  (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, kRowsStage>, at, lds);
+  if constexpr (has_split<T, TR>()) {
+    (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, kRowsSplit>, at, lds);
+    (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, kRowsSplit>, at, lds);
+  }
   (void)hipFuncSetAttribute((const void*)&k_trav_dist<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
