@@ -22,6 +22,7 @@ EXPORTS = [
     "ddg_ground_truth", "ddg_sort_rows", "ddg_size", "ddg_max_degree", "ddg_dims", "ddg_aligned_dims",
     "ddg_last_profile", "ddg_prepare_traversal", "ddg_distances", "ddg_autotune_search", "ddg_beam_search",
     "ddg_graph_set_rows", "ddg_graph_get_rows", "ddg_search_device_noalloc", "ddg_quantized_mips_distances",
+    "ddg_search_shape",
 ]
 
 
@@ -91,6 +92,7 @@ def lib():
         "ddg_quantized_mips_distances": ([i32, u32, C.c_float, C.c_float, vp, i32, u64, u32, vp, u64, u32, vp], i32),
         "ddg_search_device_noalloc": ([vp, vp, u64, vp, vp, u32, P(QueryParamsC), vp, u32, vp, vp, vp, vp], i32),
         "ddg_graph_get_rows": ([vp, vp, u64, vp], i32),
+        "ddg_search_shape": ([vp, P(QueryParamsC), vp], i32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

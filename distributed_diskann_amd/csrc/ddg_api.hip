@@ -225,6 +225,39 @@ static bool global_table(uint32_t bits) {
 
 static uint32_t tune_key(int trav, uint32_t bits) { return ((uint32_t)trav << 8) | bits; }
 
+// Launch shapes the autotuner chose, for the whole process: any handle with
+// the same dtype, metric, dimensions, degree bound, traversal mode and filter
+// size (on a GPU with the same CU count) starts from them.
+struct ProcessTuned {
+  int dtype, metric;
+  uint32_t d, ad, R, key, num_cu;
+  ddg_index::Tuned cfg;
+};
+static std::mutex g_tune_mu;
+static std::vector<ProcessTuned> g_tune_table;
+
+static bool process_tuned(const ddg_index* idx, uint32_t key, ddg_index::Tuned* out) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (const auto& e : g_tune_table)
+    if (e.dtype == idx->dtype && e.metric == idx->metric && e.d == idx->d && e.ad == idx->ad && e.R == idx->R &&
+        e.key == key && e.num_cu == idx->num_cu) {
+      *out = e.cfg;
+      return true;
+    }
+  return false;
+}
+
+static void process_tune_store(const ddg_index* idx, const ddg_index::Tuned& cfg) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto& e : g_tune_table)
+    if (e.dtype == idx->dtype && e.metric == idx->metric && e.d == idx->d && e.ad == idx->ad && e.R == idx->R &&
+        e.key == cfg.key && e.num_cu == idx->num_cu) {
+      e.cfg = cfg;
+      return;
+    }
+  g_tune_table.push_back(ProcessTuned{idx->dtype, idx->metric, idx->d, idx->ad, idx->R, cfg.key, idx->num_cu, cfg});
+}
+
 // Filter sizes (log2 slots) that keep a written-slot bitmap with a global
 // filter (native traversal; the reduced-precision kernels are compiled
 // without it): 2^12 .. 2^13 slots.  Larger filters with a bitmap were
@@ -254,16 +287,24 @@ static bool search_layout_cfg(const ddg_index* idx, uint32_t beam_max, uint32_t 
                               const ddg_index::Tuned* cfg, SearchArgs& a, Launch& l);
 
 static bool search_layout(const ddg_index* idx, uint32_t beam_max, uint32_t nstarts, int trav, SearchArgs& a,
-                          Launch& l, bool use_tuned = true) {
+                          Launch& l, bool use_tuned = true, uint32_t* source = nullptr) {
+  if (source) *source = 0;
   if (idx->cfg_override) return search_layout_cfg(idx, beam_max, nstarts, trav, idx->cfg_override, a, l);
   const uint32_t bits = filter_bits(beam_max);
   const ddg_index::Tuned* cfg = nullptr;
-  if (use_tuned)
+  uint32_t src = 0;
+  ddg_index::Tuned pt{};
+  if (use_tuned) {
     for (const auto& t : idx->tuned)
-      if (t.key == tune_key(trav, bits)) cfg = &t;
+      if (t.key == tune_key(trav, bits)) { cfg = &t; src = 1; }
+    if (!cfg && process_tuned(idx, tune_key(trav, bits), &pt)) { cfg = &pt; src = 2; }
+  }
   // a shape tuned at another beam or start count of the same filter class may
   // not fit this frontier; the default shape is then used
-  if (cfg && search_layout_cfg(idx, beam_max, nstarts, trav, cfg, a, l)) return true;
+  if (cfg && search_layout_cfg(idx, beam_max, nstarts, trav, cfg, a, l)) {
+    if (source) *source = src;
+    return true;
+  }
   return search_layout_cfg(idx, beam_max, nstarts, trav, nullptr, a, l);
 }
 
@@ -1837,7 +1878,10 @@ ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq,
       if (ms < best_ms) { best_ms = ms; best = c; }
     }
   }
-  if (best.wpb) idx->tuned.push_back(best);
+  if (best.wpb) {
+    idx->tuned.push_back(best);
+    process_tune_store(idx, best);
+  }
   if (out4) {
     SearchArgs a{};
     Launch l{};
@@ -1848,6 +1892,21 @@ ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq,
     out4[3] = tried;
   }
   if (ms2) { ms2[0] = base_ms; ms2[1] = best_ms; }
+  return DDG_OK;
+}
+
+ddg_status ddg_search_shape(ddg_index* idx, const ddg_query_params* qp, uint32_t* out4) {
+  if (!idx || !qp || !out4) return fail(idx, DDG_EINVAL, "NULL argument");
+  if (qp->beam == 0) return fail(idx, DDG_EINVAL, "bad beam");
+  SearchArgs a{};
+  Launch l{};
+  uint32_t src = 0;
+  if (!search_layout(idx, qp->beam, 1, (int)qp->trav, a, l, true, &src))
+    return fail(idx, DDG_EINVAL, "beam width too large for the LDS frontier");
+  out4[0] = (a.gtable_words ? 1u : 0u) | (a.stage_rows << 8);
+  out4[1] = l.waves_per_block;
+  out4[2] = l.blocks / idx->num_cu;
+  out4[3] = src;
   return DDG_OK;
 }
 

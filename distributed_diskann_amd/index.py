@@ -210,6 +210,16 @@ class Index:
                 "blocks_per_cu": int(out[2]), "tried": int(out[3]), "default_ms": float(ms[0]),
                 "tuned_ms": float(ms[1])}
 
+    def search_shape(self, beam, trav="native"):
+        """The launch shape a search with this beam would use now: dict of
+        global_filter, staged_rows, waves_per_block, blocks_per_cu and source
+        ("default", "handle" or "process": where a tuned shape came from)."""
+        qp = QueryParamsC(0, beam, 1.35, 0, 0, TRAV[trav], 0)
+        out = np.zeros(4, _U32)
+        self._chk(lib().ddg_search_shape(self.h, C.byref(qp), _c(out)))
+        return {"global_filter": bool(out[0] & 1), "staged_rows": int(out[0] >> 8), "waves_per_block": int(out[1]),
+                "blocks_per_cu": int(out[2]), "source": ["default", "handle", "process"][int(out[3])]}
+
     def prepare_traversal(self, trav):
         """Builds the BF16 / FP8 traversal copy; returns its device seconds."""
         sec = C.c_double(0.0)

@@ -228,11 +228,21 @@ ddg_status ddg_prepare_traversal(ddg_index* idx, ddg_trav trav, double* seconds)
  * are fixed by beamSearch.h:64-73 and the beam, so they are not tuned.
  * For rows longer than 128 bytes it also tries gathering 2, 4, 8 or as
  * many candidate rows as one load batch holds into LDS per round trip.
+ * The choice is also kept in a table of the process, keyed by (dtype,
+ * metric, d, aligned d, R, traversal mode, filter size, CU count), which
+ * every handle with the same key uses until it tunes for itself.
  * out4 (nullable): [filter in global memory (bit 0) | staged rows << 8,
  * waves per block, blocks per CU, shapes tried]; ms2 (nullable): [default
  * shape ms, chosen shape ms]. */
 ddg_status ddg_autotune_search(ddg_index* idx, const void* queries, uint64_t nq, const ddg_query_params* qp,
                                uint32_t* out4, double* ms2);
+
+/* The launch shape a search of this handle with qp (beam qp->beam, one
+ * start point) would use now, in ddg_autotune_search's out4 layout except
+ * out4[3]: where the shape comes from (0 the defaults, 1 this handle's own
+ * tuning, 2 the process table).  No reference counterpart: it lets a caller
+ * (and the tests) see what the autotuner left behind. */
+ddg_status ddg_search_shape(ddg_index* idx, const ddg_query_params* qp, uint32_t* out4);
 
 /* Point::distance (euclidian_point.h:44-65, mips_point.h:44-66) batched, in
  * the arithmetic of traversal mode trav: out[i*m + j] = distance of query i

@@ -73,6 +73,36 @@ def test_autotune_search(oracle, xs, Ls):
     assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
 
 
+def test_autotune_process_table(oracle, xs):
+    """A tuned shape is kept for the whole process: a second handle over the
+    same kind of index (dtype, metric, dimensions, degree bound) starts from
+    it, with unchanged results; a handle with another degree bound does not."""
+    from distributed_diskann_amd import Index
+    print("[gpu] autotune process table", flush=True)
+    Ls = 150  # a filter class no other test tunes
+    a = _index(xs["X"], xs["d"], xs["G"])
+    assert a.search_shape(Ls)["source"] == "default"
+    info = a.autotune_search(xs["Q"], 10, Ls)
+    sa = a.search_shape(Ls)
+    assert sa["source"] == "handle"
+    assert (sa["waves_per_block"], sa["blocks_per_cu"]) == (info["waves_per_block"], info["blocks_per_cu"])
+    b = _index(xs["X"], xs["d"], xs["G"])
+    sb = b.search_shape(Ls)
+    print(f"[gpu] tuned {sa}, second handle {sb}", flush=True)
+    assert sb["source"] == "process"
+    assert {k: v for k, v in sb.items() if k != "source"} == {k: v for k, v in sa.items() if k != "source"}
+    ids, dists, cmps, vis = b.search(xs["Q"], 10, Ls, 1.35)
+    P = oracle.Points(xs["X"], xs["d"])
+    oi, od, oc, ov = oracle.search_all(P, oracle.Graph(xs["G"]), xs["Q"], 10, Ls, cut=1.35)
+    assert np.array_equal(ids, oi) and np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+    assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+    R2 = xs["G"].shape[1] - 1 + 8
+    c = Index(xs["X"], xs["d"], "l2", R2)
+    assert c.search_shape(Ls)["source"] == "default"
+    for h in (a, b, c):
+        h.close()
+
+
 @pytest.mark.parametrize("stage", [2, 9])
 @pytest.mark.parametrize("dtype,metric,d", [("float32", "l2", 200), ("float32", "mips", 150)])
 def test_search_staged_rows(oracle, dtype, metric, d, stage):
