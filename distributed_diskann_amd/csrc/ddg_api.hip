@@ -33,7 +33,7 @@ enum BufId {
   B_QUERIES, B_STARTS, B_QBEAM, B_QIDS, B_OUT_IDS, B_OUT_D, B_CMPS, B_VISN, B_VIS_IDS, B_VIS_D, B_FLAGS,
   B_INSERTS, B_NEW_IDS, B_NEW_LEN, B_OFFS, B_KEYS, B_VALS, B_KEYS2, B_VALS2, B_TMP, B_GFLAGS, B_GOFFS, B_GKEYS,
   B_GOFF, B_BIG, B_GSD, B_GSI, B_VC, B_PRUNE_OFF, B_PRUNE_IDS, B_PRUNE_D, B_GT_D, B_GTAB, B_BIGOFF, B_RPTS,
-  B_FR_IDS, B_FR_D, B_FR_LEN, B_ROW_IDS, B_ROWS, B_GEPOCH, B_COUNT
+  B_FR_IDS, B_FR_D, B_FR_LEN, B_ROW_IDS, B_ROWS, B_GEPOCH, B_OKEYS, B_OVALS, B_OKEYS2, B_OVALS2, B_OTMP, B_COUNT
 };
 static_assert(B_COUNT <= 48, "scratch slots");
 
@@ -1438,6 +1438,64 @@ struct SearchExtra {
   bool overflow = false;  // out: some visited list outgrew vcap
 };
 
+// The order a launch takes its queries in (a.order, task t -> query): the
+// longest expected first, so that the launch's end is not set by a long
+// query that started late.  With per-query beams the key is the beam (the
+// work grows with it): 100K x 20 with beams {10..200}, 1.70 -> 1.40 ms.
+// With DDG_ORDER=start, and more queries than resident waves, uniform-beam
+// launches are ordered by the distance to the start point (larger first), a
+// weak predictor of a query's length: 10M x 128 uint8 at Ls=140 3.45 ->
+// 3.20 ms per launch including the ordering, but 10M x 200 MIPS FP8 at
+// Ls=250 unchanged (3.52 ms) and native FP32 at Ls=275 slower (5.88 -> 5.98
+// ms); so it is not the default.  Keys sort ascending and stably, so equal keys keep the queries'
+// own order.  Results do not depend on the order.  Allocates: not for the
+// allocation-free path.  DDG_ORDER=0 keeps the queries in their own order.
+__global__ void k_beam_order_keys(const uint32_t* beams, uint32_t beam_max, uint64_t nq, uint32_t* keys,
+                                  uint32_t* vals) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const uint32_t b = beams[i];
+  keys[i] = beam_max - (b < 1u ? 1u : (b < beam_max ? b : beam_max));
+  vals[i] = (uint32_t)i;
+}
+
+static ddg_status order_queries(ddg_index* idx, int trav, const uint32_t* beams, uint32_t beam_max, uint64_t nq,
+                                const Launch& l, SearchArgs& a) {
+  a.order = nullptr;
+  const char* e = getenv("DDG_ORDER");
+  if (nq < 2 || (e && e[0] == '0')) return DDG_OK;
+  const bool by_start = e && strcmp(e, "start") == 0;
+  if (!beams && (!by_start || nq <= (uint64_t)l.blocks * l.waves_per_block)) return DDG_OK;
+  ENSURE(B_OKEYS, nq * 4);
+  ENSURE(B_OVALS, nq * 4);
+  ENSURE(B_OKEYS2, nq * 4);
+  ENSURE(B_OVALS2, nq * 4);
+  ENSURE(B_OTMP, radix_tmp_bytes(nq) + 64);
+  uint32_t kb = 16;
+  if (beams) {
+    hipLaunchKernelGGL(k_beam_order_keys, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, idx->stream, beams,
+                       beam_max, nq, buf<uint32_t>(idx, B_OKEYS), buf<uint32_t>(idx, B_OVALS));
+    HIPCHK(hipGetLastError());
+    kb = 1;
+    while (kb < 32 && (1ull << kb) <= beam_max) kb++;
+  } else {
+    SearchArgs k = a;
+    k.lds_per_wave = trav == DDG_TRAV_FP8E4M3 ? 2 * a.qrow_bytes : a.qrow_bytes;
+    Launch kl{};
+    kl.waves_per_block = 4;
+    kl.lds_bytes = 4 * k.lds_per_wave;
+    kl.blocks = (uint32_t)((nq + 3) / 4);
+    HIPCHK(launch_start_keys(idx->dtype, idx->metric, trav, k, buf<uint32_t>(idx, B_OKEYS),
+                             buf<uint32_t>(idx, B_OVALS), kl, idx->stream));
+  }
+  bool alt = false;
+  HIPCHK(radix_sort_pairs(buf<uint32_t>(idx, B_OKEYS), buf<uint32_t>(idx, B_OVALS), buf<uint32_t>(idx, B_OKEYS2),
+                          buf<uint32_t>(idx, B_OVALS2), nq, kb, buf<void>(idx, B_OTMP), idx->s.b[B_OTMP].cap,
+                          idx->stream, &alt));
+  a.order = alt ? buf<uint32_t>(idx, B_OVALS2) : buf<uint32_t>(idx, B_OVALS);
+  return DDG_OK;
+}
+
 // One search launch over device-resident queries and outputs; the caller
 // has checked every argument.
 // `fixed` (nullable): a caller-provided scratch of fixed_bytes that takes the
@@ -1519,6 +1577,8 @@ static ddg_status search_launch(ddg_index* idx, const void* queries, uint64_t nq
     }
   } else {
     ENSURE_GTABLE(a, l);
+    ddg_status ost = order_queries(idx, trav, per_query_beam, beam_max, nq, l, a);
+    if (ost != DDG_OK) return ost;
   }
   Timer t(idx, &idx->prof[0]);
   HIPCHK(launch_search(idx->dtype, idx->metric, trav, a, l, idx->stream));

@@ -23,6 +23,7 @@ struct SearchArgs {
   const uint32_t* qids;          // build mode: point id per task; query mode: p.id() (nullable)
   const uint32_t* starts;
   const uint32_t* per_query_beam;  // nullable
+  const uint32_t* order;         // nullable: task t serves query order[t] (longest expected first)
   uint32_t* out_ids;             // nq x k (query mode)
   float* out_dists;              // nullable
   uint64_t* dist_cmps;           // nullable, nq
@@ -111,6 +112,11 @@ hipError_t launch_search(int dtype, int metric, int trav, const SearchArgs& a, c
 // one wave per query, `a` carries the rows, copies and LDS layout
 hipError_t launch_trav_dist(int dtype, int metric, int trav, const SearchArgs& a, const uint32_t* ids, uint32_t m,
                             float* out, const Launch& l, hipStream_t s);
+// ordering keys of a launch's queries (distance to the start point, larger
+// first, 16 bits), vals[q] = q; one wave per query, `a` carries the queries,
+// the start point, the copies and the staged-query LDS layout
+hipError_t launch_start_keys(int dtype, int metric, int trav, const SearchArgs& a, uint32_t* keys, uint32_t* vals,
+                             const Launch& l, hipStream_t s);
 // reduced traversal copies of float32 rows (trav.hip)
 hipError_t launch_col_absmax(const float* rows, uint64_t n, uint32_t ad, uint32_t d, uint32_t* maxbits,
                              hipStream_t s);

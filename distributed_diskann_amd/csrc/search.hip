@@ -156,8 +156,9 @@ void k_search(SearchArgs a) {
   const T* base = (const T*)a.base;
 
   for (;;) {
-    const uint32_t q = fetch_task(a.work);
-    if (q >= a.nq) break;
+    const uint32_t t = fetch_task(a.work);
+    if (t >= a.nq) break;
+    const uint32_t q = a.order ? a.order[t] : t;  // the query this task serves
 
     // a per-query beam is clamped to [1, beam]: the frontier and the filter
     // were sized for the largest beam the host declared
@@ -741,6 +742,57 @@ __global__ __launch_bounds__(256) void k_trav_dist(SearchArgs a, const uint32_t*
   }
 }
 
+// Ordering keys of a launch's queries: the distance of query q to the start
+// point in the traversal's own arithmetic, larger first (a query far from
+// the start tends to need more expansions), as a 16-bit key that sorts
+// ascending: the top half of the complemented order-preserving bits.  One
+// wave per query; vals[q] = q.
+template <typename T, int METRIC, int TRAV>
+__global__ __launch_bounds__(256) void k_start_keys(SearchArgs a, uint32_t* keys, uint32_t* vals) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  using P = TravDist<T, METRIC, TRAV>;
+  const uint32_t wv = bcast0(threadIdx.x >> 6);
+  unsigned char* ql = smem + wv * a.lds_per_wave;
+  const uint64_t q = (uint64_t)blockIdx.x * (blockDim.x >> 6) + wv;
+  if (q >= a.nq) return;  // whole waves leave together
+  stage_traversal_query<T, METRIC, TRAV>(a, ql, (const T*)a.queries + q * a.ad);
+  wave_sync();
+  const typename P::Q qs = P::prep(a, ql);
+  const uint32_t sid = a.starts[0];
+  const float d = sid < a.n ? P::dist(a, sid, ql, qs) : 0.f;
+  if (lane_id() == 0) {
+    const uint32_t b = __float_as_uint(d);
+    const uint32_t ord = b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);  // ascending with d
+    keys[q] = (~ord) >> 16;
+    vals[q] = (uint32_t)q;
+  }
+}
+
+template <typename T, int M, int TR>
+static hipError_t launch_k(const SearchArgs& a, uint32_t* keys, uint32_t* vals, const Launch& l, hipStream_t s) {
+  hipLaunchKernelGGL((k_start_keys<T, M, TR>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s, a,
+                     keys, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_start_keys(int dtype, int metric, int trav, const SearchArgs& a, uint32_t* keys, uint32_t* vals,
+                             const Launch& l, hipStream_t s) {
+  if (trav == TRAV_BF16)
+    return metric == L2 ? launch_k<float, L2, TRAV_BF16>(a, keys, vals, l, s)
+                        : launch_k<float, MIPS, TRAV_BF16>(a, keys, vals, l, s);
+  if (trav == TRAV_FP8)
+    return metric == L2 ? launch_k<float, L2, TRAV_FP8>(a, keys, vals, l, s)
+                        : launch_k<float, MIPS, TRAV_FP8>(a, keys, vals, l, s);
+  if (dtype == U8)
+    return metric == L2 ? launch_k<uint8_t, L2, TRAV_NATIVE>(a, keys, vals, l, s)
+                        : launch_k<uint8_t, MIPS, TRAV_NATIVE>(a, keys, vals, l, s);
+  if (dtype == I8)
+    return metric == L2 ? launch_k<int8_t, L2, TRAV_NATIVE>(a, keys, vals, l, s)
+                        : launch_k<int8_t, MIPS, TRAV_NATIVE>(a, keys, vals, l, s);
+  return metric == L2 ? launch_k<float, L2, TRAV_NATIVE>(a, keys, vals, l, s)
+                      : launch_k<float, MIPS, TRAV_NATIVE>(a, keys, vals, l, s);
+}
+
 template <typename T, int M, int TR, bool G, int RW>
 static hipError_t launch_g(const SearchArgs& a, const Launch& l, hipStream_t s) {
   hipLaunchKernelGGL((k_search<T, M, TR, G, RW>), dim3(l.blocks), dim3(l.waves_per_block * kWave), l.lds_bytes, s,
@@ -783,6 +835,7 @@ static void set_attrs_t(int lds) {
     (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, kRowsSplit>, at, lds);
   }
   (void)hipFuncSetAttribute((const void*)&k_trav_dist<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  (void)hipFuncSetAttribute((const void*)&k_start_keys<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
 }
 
 // Raises the dynamic-LDS limit of every search kernel once (outside any

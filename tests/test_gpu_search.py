@@ -333,3 +333,29 @@ def test_ground_truth_vs_oracle(oracle, xs):
         ids, d = idx2.ground_truth(Qa, 20)
         oi, od = oracle.ground_truth(P, Qa, 20)
         assert np.array_equal(ids, oi) and np.array_equal(d, od)
+
+
+@pytest.mark.parametrize("order", ["start", "0"])
+def test_search_query_order(oracle, monkeypatch, order):
+    """The order a launch takes its queries in (largest beam first by
+    default; by distance to the start point with DDG_ORDER=start; none with
+    DDG_ORDER=0) changes no result: 12,000 queries (more than the resident
+    waves) bit-exact against the oracle, with uniform and per-query beams."""
+    from distributed_diskann_amd.formats import align_rows
+    print(f"[gpu] query order {order}", flush=True)
+    monkeypatch.setenv("DDG_ORDER", order)
+    rng = np.random.default_rng(77)
+    d = 20
+    X = align_rows(rng.standard_normal((20000, d)).astype(np.float32))
+    Q = align_rows(rng.standard_normal((12000, d)).astype(np.float32))
+    P = oracle.Points(X, d)
+    G = oracle.build(P, 16, 32, 1.2, tie_mode=oracle.TIE_ID)
+    idx = _index(X, d, G.rows)
+    beams = rng.choice(np.array([10, 20, 50, 100], np.uint32), Q.shape[0])
+    for pqb in (None, beams):
+        ids, dists, cmps, vis = idx.search(Q, 10, 100, 1.35, per_query_beam=pqb)
+        oi, od, oc, ov = oracle.search_all(P, G, Q, 10, 100, cut=1.35, per_query_beam=pqb)
+        assert np.array_equal(ids, oi)
+        assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
+        assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+    idx.close()
