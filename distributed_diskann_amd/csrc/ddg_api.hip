@@ -439,14 +439,15 @@ static int env_int(const char* name, int dflt, int lo, int hi) {
 }
 // Defaults from a 10M x 128 uint8 build sweep: 12 KB per wave; forward
 // prunes keep up to 512 candidates in LDS (visited lists), reverse prunes
-// 128 (|S| + R for most targets; hub targets with thousands of new
+// 128 or 256 (|S| + R for most targets; hub targets with thousands of new
 // in-edges take the global-scratch pass instead of inflating every wave).
-// Waves of a block of the queued-task pass (1, 2, 4, 8 or 16).  8 measured
-// best over the 10M x 128 uint8 and 1M x 100 int8 builds (4 suits the int8
-// build's shorter lists, 16 the uint8 build's hubs).
+// Waves of a block of the queued-task pass (1, 2, 4, 8 or 16).  16 measured
+// best over the 10M x 128 uint8, 1M x 100 int8 and 10M x 200 float32 builds
+// with the reverse caps below (uint8 reverse 801 -> 779 ms against 8): the
+// queued tasks are hubs whose greedy steps are a serial chain per block.
 static uint32_t prune_big_waves() {
-  static const int w = env_int("DDG_PRUNE_BIG_WAVES", 8, 1, (int)kMaxBigWaves);
-  return (uint32_t)(w & (w - 1) ? 8 : w);
+  static const int w = env_int("DDG_PRUNE_BIG_WAVES", 16, 1, (int)kMaxBigWaves);
+  return (uint32_t)(w & (w - 1) ? 16 : w);
 }
 static uint32_t prune_wave_lds() {
   static const int kb = env_int("DDG_PRUNE_LDS", 12, 0, 160);
@@ -456,12 +457,22 @@ static uint32_t prune_ccap_max(uint32_t dflt) {
   static const int c = env_int("DDG_PRUNE_CCAP", 0, 64, 4096);
   return c ? (uint32_t)c : dflt;
 }
-constexpr uint32_t kForwardCcap = 512, kReverseCcap = 128, kAnyCcap = 4096;
+// Reverse lists of up to 256 candidates stay in the wave's LDS when the rows
+// prune from the Gram matrix (integer rows): 1M x 100 int8 reverse 74 -> 43
+// ms, 10M x 128 uint8 826 -> 801 ms against 128; float32 rows keep 128
+// (10M x 200 MIPS 300 -> 316 ms at 256).
+constexpr uint32_t kForwardCcap = 512, kReverseCcap = 128, kReverseCcapGram = 256, kAnyCcap = 4096;
 // DDG_PRUNE_GRAM=0 prunes integer rows pair by pair instead of from the
 // Gram matrix (measurement only; the same selections)
 static bool prune_gram_enabled() {
   static const int on = env_int("DDG_PRUNE_GRAM", 1, 0, 1);
   return on != 0;
+}
+// integer rows of at most 256 bytes, a multiple of 64: the Gram-matrix prune
+// applies (its tiles split each row into four lane groups of whole 16-byte K
+// slabs, which covers the row only when it is a multiple of 64 bytes)
+static bool prune_gram_rows(const ddg_index* idx) {
+  return idx->es == 1 && idx->ad <= 256 && idx->ad % 64 == 0 && prune_gram_enabled();
 }
 
 // Per-wave LDS of a prune launch: p's and p*'s rows, the selection, ccap
@@ -481,13 +492,11 @@ static void prune_layout(const ddg_index* idx, uint32_t Rsel, uint32_t want_ccap
   a.ccap = ccap;
   a.off_cand = 2 * row + round_up(Rsel * 4, 16);
   a.off_rows = fixed(ccap);
-  // integer rows of at most 256 bytes: tasks of up to 256 candidates prune
-  // from an MFMA Gram matrix (its LDS replaces the row cache).  Its tiles
-  // split each row into four lane groups of whole 16-byte K slabs, which
-  // covers the row only when it is a multiple of 64 bytes.
+  // Gram rows: tasks of up to 256 candidates prune from an MFMA Gram matrix
+  // (its LDS replaces the row cache)
   a.gram_max = 0;
   a.off_gram = fixed(ccap);
-  if (idx->es == 1 && idx->ad <= 256 && idx->ad % 64 == 0 && prune_gram_enabled()) {
+  if (prune_gram_rows(idx)) {
     a.gram_max = std::min<uint32_t>(ccap, 256);
     a.rrows = 0;
     a.lds_per_wave = round_up(fixed(ccap) + prune_gram_lds_bytes(), 16);
@@ -843,7 +852,7 @@ static ddg_status round_prepare(ddg_index* idx, const RoundShape& r, PrunePlan& 
   rev.a.add = 1;
   rev.a.write_graph = 1;
   rev.a.append_if_fits = 1;
-  uint32_t rcap = kReverseCcap;
+  uint32_t rcap = prune_gram_rows(idx) ? kReverseCcapGram : kReverseCcap;
   while (rcap < 2 * idx->R) rcap <<= 1;
   return prune_prepare(idx, rev, Pmax, rcap + 1 + idx->R, Pmax, rcap, false);
 }

@@ -260,38 +260,154 @@ __device__ __forceinline__ float int_row_terms(const uint4* r, const T* q, const
   }
 }
 
+// Sortable key of a candidate: the distance's bits sign-folded over the id
+// (zero taken as +0.0), so the unsigned order of keys is the (distance, id)
+// order of less_di.
+__device__ __forceinline__ uint64_t prune_key(float d, uint32_t id) {
+  uint32_t b = __float_as_uint(d);
+  if ((b << 1) == 0u) b = 0u;
+  b ^= (uint32_t)((int32_t)b >> 31) | 0x80000000u;
+  return ((uint64_t)b << 32) | id;
+}
+
+// rk[u] += the number of keys of K[0, M4) below ku[u]; K is padded to M4 (a
+// multiple of 4) with keys above every candidate's.
+template <uint32_t NS>
+__device__ __forceinline__ void count_below_keys(const uint64_t* K, uint32_t M4, const uint64_t* ku, uint32_t* rk) {
+  for (uint32_t jj = 0; jj < M4; jj += 2) {
+    const ulonglong2 e = *(const ulonglong2*)(K + jj);
+#pragma unroll
+    for (uint32_t u = 0; u < NS; u++) rk[u] += (uint32_t)(e.x < ku[u]) + (uint32_t)(e.y < ku[u]);
+  }
+}
+
+// The largest integer x with alpha * (double)x <= c (alpha > 0), clamped to
+// +-2^30: the prune test of index.h:97-103 for an integer distance x, which
+// (float)x holds exactly below 2^24.
+__device__ __forceinline__ int32_t prune_threshold(double alpha, double c) {
+  constexpr double kLim = 1073741824.0;
+  double f = floor(c / alpha);
+  f = f < -kLim ? -kLim : (f > kLim ? kLim : f);
+  while (f < kLim && alpha * (f + 1.0) <= c) f += 1.0;
+  while (f > -kLim && !(alpha * f <= c)) f -= 1.0;
+  return (int32_t)f;
+}
+
+// The upper triangle of the Gram matrix of the sorted candidates in 16 x 16
+// tiles: v_mfma_i32_16x16x64_i8 leaves row 4 (lane / 16) + r, column
+// lane % 16 in accumulator register r; lane (g4, c16) supplies the 16 bytes
+// at chunk g4 NST + s of its row for K step s (the same K order for both
+// operands).  Tile pairs (I, J >= I) run in row-major order and each needs
+// one block of 16 rows (J's): the blocks are loaded kRing pairs ahead into a
+// ring of registers (loads past the last pair re-read the last block, in
+// bounds), and row block I's operand is the block of pair (I, I).
+//
+// Row a prunes column b when its integer distance RA[a] - K acc + CB[b] is
+// at most b's threshold (alpha * d(a, b) <= d(p, b)), i.e. when
+// RA[a] - K acc <= Tb[b] (K = 2 for L2, 1 for MIPS; RA, CB hold the norms and
+// the uint8 offset terms); the tile's 16 x 16 bits go to the bit matrix as
+// one 16-bit word per row.  Only words of blocks J >= I are written: bits
+// of b before a are never read, since a is applied only when every earlier
+// position is settled.
+template <typename T, int METRIC, uint32_t NST>
+__device__ __forceinline__ void gram_tiles(const PruneArgs& a, const uint32_t* Ci, const int32_t* RA,
+                                           const int32_t* Tb, uint16_t* Pm16, uint32_t Mp) {
+  constexpr bool U8 = sizeof(T) == 1 && T(-1) > T(0);
+  constexpr uint32_t kFlip = U8 ? 0x80808080u : 0u;
+  constexpr uint32_t kRing = 4;
+  constexpr int32_t K = METRIC == L2 ? 2 : 1;
+  const T* base = (const T*)a.base;
+  const uint32_t lane = lane_id();
+  const uint32_t g4 = lane >> 4, c16 = lane & 15u;
+  const uint32_t nb = (Mp + 15u) / 16u, npairs = nb * (nb + 1u) / 2u;
+  auto load_block = [&](uint32_t J, uint4* v) {
+    const uint32_t Jc = J < nb ? J : nb - 1u;
+    const uint32_t rb = 16u * Jc + c16 < Mp ? 16u * Jc + c16 : Mp - 1u;
+    const uint4* rc = (const uint4*)(base + (uint64_t)Ci[rb] * a.ad) + g4 * NST;
+#pragma unroll
+    for (uint32_t st = 0; st < NST; st++) v[st] = rc[st];
+  };
+  auto frag = [&](const uint4& v) {
+    return v4i_t{(int)(v.x ^ kFlip), (int)(v.y ^ kFlip), (int)(v.z ^ kFlip), (int)(v.w ^ kFlip)};
+  };
+  auto next = [&](uint32_t& I, uint32_t& J) {
+    if (++J >= nb) J = ++I;
+  };
+  uint4 ring[kRing][NST];
+  uint32_t Il = 0, Jl = 0;  // the next pair to load
+#pragma unroll
+  for (uint32_t s = 0; s < kRing; s++) {
+    load_block(Jl, ring[s]);
+    next(Il, Jl);
+  }
+  uint32_t I = 0, J = 0;  // the next pair to multiply
+  v4i_t af[NST];
+  int32_t ra[4];
+  for (uint32_t k0 = 0; k0 < npairs; k0 += kRing) {
+#pragma unroll
+    for (uint32_t s = 0; s < kRing; s++) {
+      if (k0 + s < npairs) {
+        if (J == I) {
+#pragma unroll
+          for (uint32_t st = 0; st < NST; st++) af[st] = frag(ring[s][st]);
+#pragma unroll
+          for (uint32_t r = 0; r < 4; r++) ra[r] = RA[16u * I + 4u * g4 + r];
+        }
+        v4i_t acc = v4i_t{0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t st = 0; st < NST; st++)
+          acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[st], frag(ring[s][st]), acc, 0, 0, 0);
+        const int32_t tb = Tb[16u * J + c16];
+        uint64_t m[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; r++) m[r] = ballot(ra[r] - K * acc[r] <= tb);
+        if (lane < 16u) {  // lane L writes row 16 I + L: bits 16 (L / 4) .. of m[L % 4]
+          const uint32_t rr = lane & 3u;
+          const uint64_t mm = rr == 0u ? m[0] : (rr == 1u ? m[1] : (rr == 2u ? m[2] : m[3]));
+          Pm16[(16u * I + lane) * (kGramMax / 16u) + J] = (uint16_t)(mm >> (16u * (lane >> 2)));
+        }
+        next(I, J);
+      }
+      load_block(Jl, ring[s]);
+      next(Il, Jl);
+    }
+  }
+}
+
 template <typename T, int METRIC>
 __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, uint32_t p, uint32_t* grow,
                                                 uint64_t a0, uint64_t alen, uint32_t gdeg, T* pv, uint32_t* sel,
                                                 float* Cd, uint32_t* Ci, unsigned char* gram) {
   using D = Dist<T, METRIC>;
   constexpr bool U8 = sizeof(T) == 1 && T(-1) > T(0);
-  constexpr uint32_t kFlip = U8 ? 0x80808080u : 0u;
+  constexpr uint32_t kSlots = kGramMax / kWave;
   const uint32_t lane = lane_id();
   const uint32_t row_bytes = a.ad * (uint32_t)sizeof(T);
   const uint32_t nchunks = row_bytes >> 4;
   const T* base = (const T*)a.base;
-  const uint32_t M = (uint32_t)(alen + gdeg);
-  int32_t* Nrm = (int32_t*)gram;
+  const uint32_t M = (uint32_t)(alen + gdeg), M4 = (M + 3u) & ~3u;
+  int32_t* Nrm = (int32_t*)gram;  // norms, then the row terms RA
   int32_t* Sm = Nrm + kGramMax;
   uint32_t* Pm = (uint32_t*)(Sm + kGramMax);
-  // until the sort is done, the bit matrix's space holds each candidate's
-  // norm and sum by list position, and the sort carries that position
-  int32_t* NrmL = (int32_t*)Pm;
+  int32_t* Tb = (int32_t*)Cd;  // the column thresholds replace the distances
+  // until the ranks are known, the bit matrix's space holds each candidate's
+  // key, norm, sum and a tie counter by list position
+  uint64_t* Key = (uint64_t*)Pm;
+  int32_t* NrmL = (int32_t*)(Key + kGramMax);
   int32_t* SmL = NrmL + kGramMax;
-  uint16_t* Pos = (uint16_t*)(SmL + kGramMax);
+  uint32_t* Tie = (uint32_t*)(SmL + kGramMax);
   stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
   for (uint32_t j = lane; j < M; j += kWave) Ci[j] = j < alen ? a.a_ids[a0 + j] : grow[1 + (j - (uint32_t)alen)];
   wave_sync();
   const typename D::Q ps = D::prep(pv, nchunks);
   // one pass over each candidate row: distance to p (unless given), norm and
-  // (uint8) byte sum; candidates equal to p sort last and are never taken
-  uint32_t P2 = 1;
-  while (P2 < M) P2 <<= 1;
-  for (uint32_t j = lane; j < P2; j += kWave) {
+  // (uint8) byte sum; candidates equal to p get (inf, kPruned), last in the
+  // order, never taken
+  for (uint32_t j = lane; j < M4; j += kWave) {
     float dj = __builtin_inff();
     uint32_t ij = kPruned;
     int32_t nn = 0, ss = 0;
+    uint64_t key = ~0ull;  // padding: after every candidate
     if (j < M) {
       ij = Ci[j];
       if (ij == p) {
@@ -300,114 +416,97 @@ __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, 
         const float dc = int_row_terms<T, METRIC>((const uint4*)(base + (uint64_t)ij * a.ad), pv, ps, nchunks, nn, ss);
         dj = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : dc;
       }
+      key = prune_key(dj, ij);
+      Cd[j] = dj;
+      Ci[j] = ij;
     }
-    Cd[j] = dj;
-    Ci[j] = ij;
-    Pos[j] = (uint16_t)j;
+    Key[j] = key;
     NrmL[j] = nn;
     SmL[j] = ss;
+    Tie[j] = 0u;
   }
   wave_sync();
-  // bitonic sort by (distance, id)
-  for (uint32_t kk = 2; kk <= P2; kk <<= 1) {
-    for (uint32_t jj = kk >> 1; jj > 0; jj >>= 1) {
-      for (uint32_t i = lane; i < (P2 >> 1); i += kWave) {
-        const uint32_t lo = ((i / jj) * (jj << 1)) + (i % jj);
-        const uint32_t hi = lo + jj;
-        const bool up = (lo & kk) == 0;
-        const float dl = Cd[lo], dh = Cd[hi];
-        const uint32_t il = Ci[lo], ih = Ci[hi];
-        const bool sw = up ? less_di(dh, ih, dl, il) : less_di(dl, il, dh, ih);
-        if (sw) {
-          const uint16_t pl = Pos[lo], ph = Pos[hi];
-          Cd[lo] = dh; Cd[hi] = dl; Ci[lo] = ih; Ci[hi] = il; Pos[lo] = ph; Pos[hi] = pl;
-        }
-      }
-      wave_sync();
+  // sorted position = number of smaller keys; copies of one candidate (the
+  // same id, row and distance, so interchangeable) take consecutive positions
+  // in any order
+  uint64_t ku[kSlots];
+  uint32_t rk[kSlots];
+#pragma unroll
+  for (uint32_t u = 0; u < kSlots; u++) {
+    const uint32_t j = lane + u * kWave;
+    ku[u] = j < M ? Key[j] : ~0ull;
+    rk[u] = 0;
+  }
+  const uint32_t nslots = (M + kWave - 1u) / kWave;
+  if (nslots <= 1) count_below_keys<1>(Key, M4, ku, rk);
+  else if (nslots == 2) count_below_keys<2>(Key, M4, ku, rk);
+  else if (nslots == 3) count_below_keys<3>(Key, M4, ku, rk);
+  else count_below_keys<4>(Key, M4, ku, rk);
+  float dv[kSlots];
+  uint32_t iv[kSlots];
+  int32_t nv[kSlots], sv[kSlots];
+#pragma unroll
+  for (uint32_t u = 0; u < kSlots; u++) {
+    const uint32_t j = lane + u * kWave;
+    if (j < M) {
+      rk[u] += atomicAdd(&Tie[rk[u]], 1u);
+      dv[u] = Cd[j];
+      iv[u] = Ci[j];
+      nv[u] = NrmL[j];
+      sv[u] = SmL[j];
     }
   }
+  wave_sync();
+#pragma unroll
+  for (uint32_t u = 0; u < kSlots; u++) {
+    if (lane + u * kWave < M) {
+      Cd[rk[u]] = dv[u];
+      Ci[rk[u]] = iv[u];
+      Nrm[rk[u]] = nv[u];
+      Sm[rk[u]] = sv[u];
+    }
+  }
+  wave_sync();
   uint32_t Mp = 0;  // candidates other than p
   for (uint32_t b0 = 0; b0 < M; b0 += kWave) Mp += popc(ballot(b0 + lane < M && Ci[b0 + lane] != kPruned));
-  // norms and sums in sorted order, then the bit matrix cleared
-  for (uint32_t j = lane; j < Mp; j += kWave) {
-    const uint32_t o = Pos[j];
-    Nrm[j] = NrmL[o];
-    Sm[j] = SmL[o];
-  }
-  wave_sync();
-  for (uint32_t i = lane; i < Mp * kGramWords; i += kWave) Pm[i] = 0u;
-  wave_sync();
-
-  // the upper triangle of the Gram matrix in 16 x 16 tiles; v_mfma_i32_16x16x64_i8
-  // leaves row 4 (lane / 16) + r, column lane % 16 in accumulator register r;
-  // lane (g4, c16) supplies 16 bytes at element g4 * Q + 16 s of its row for
-  // K step s (the same K order for both operands, so a block's rows serve as
-  // either).  The rows of column block J + 1 are loaded while block J is
-  // multiplied; block I + 1's rows then become the next row block.
-  const uint32_t g4 = lane >> 4, c16 = lane & 15u;
-  const uint32_t Q = a.ad / 4u, nsteps = Q / 16u;
+  // row terms RA and column thresholds Tb (d(a, b) = RA[a] - K acc + CB[b],
+  // acc the dot product of the offset rows; Tb = threshold(d(p, b)) - CB[b]);
+  // columns past the candidates never prune
   const uint32_t nb = (Mp + 15u) / 16u;
-  auto load_block = [&](uint32_t J, uint4* v) {
-    const uint32_t rb = 16u * J + c16 < Mp ? 16u * J + c16 : Mp - 1u;
-    const unsigned char* rowb = (const unsigned char*)(base + (uint64_t)Ci[rb] * a.ad);
-#pragma unroll
-    for (uint32_t st = 0; st < 4; st++) v[st] = st < nsteps ? *(const uint4*)(rowb + g4 * Q + 16u * st) : make_uint4(0u, 0u, 0u, 0u);
-  };
-  auto frag = [&](const uint4& v) {
-    return v4i_t{(int)(v.x ^ kFlip), (int)(v.y ^ kFlip), (int)(v.z ^ kFlip), (int)(v.w ^ kFlip)};
-  };
-  uint4 araw[4];
-  if (nb > 0) load_block(0, araw);
-  for (uint32_t I = 0; I < nb; I++) {
-    v4i_t af[4];
-#pragma unroll
-    for (uint32_t st = 0; st < 4; st++) af[st] = frag(araw[st]);
-    int32_t na[4], sa[4];
-#pragma unroll
-    for (uint32_t r = 0; r < 4; r++) {
-      const uint32_t ar = 16u * I + 4u * g4 + r;
-      na[r] = ar < Mp ? Nrm[ar] : 0;
-      sa[r] = ar < Mp ? Sm[ar] : 0;
-    }
-    uint4 bcur[4], bnext[4];
-#pragma unroll
-    for (uint32_t st = 0; st < 4; st++) bcur[st] = araw[st];
-    for (uint32_t J = I; J < nb; J++) {
-      if (J + 1u < nb) load_block(J + 1u, bnext);
-      v4i_t acc = v4i_t{0, 0, 0, 0};
-#pragma unroll
-      for (uint32_t st = 0; st < 4; st++)
-        if (st < nsteps) acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[st], frag(bcur[st]), acc, 0, 0, 0);
-      const uint32_t b = 16u * J + c16;
-      const int32_t nbv = b < Mp ? Nrm[b] : 0, sbv = b < Mp ? Sm[b] : 0;
-      const double cdb = b < Mp ? (double)Cd[b] : 0.0;
-#pragma unroll
-      for (uint32_t r = 0; r < 4; r++) {
-        const uint32_t ar = 16u * I + 4u * g4 + r;
-        int32_t dot = acc[r];
-        if (U8) dot += 128 * (sa[r] + sbv) - 16384 * (int32_t)a.ad;
-        const float dab = METRIC == L2 ? (float)(int32_t)(na[r] + nbv - 2 * dot) : -((float)dot);
-        const bool pr = ar < Mp && b < Mp && b > ar && a.alpha * (double)dab <= cdb;
-        const uint64_t m = ballot(pr);
-        if (c16 == 0) {
-          const uint32_t bits = (uint32_t)(m >> (16u * g4)) & 0xFFFFu;
-          if (bits) atomicOr(&Pm[ar * kGramWords + (J >> 1)], bits << ((J & 1u) * 16u));
-        }
+  for (uint32_t j = lane; j < 16u * nb; j += kWave) {
+    int32_t ra = 0, tb = INT32_MIN;
+    if (j < Mp) {
+      const int32_t nn = Nrm[j], ss = Sm[j];
+      const int32_t off = U8 ? 128 * ss - 16384 * (int32_t)a.ad : 0;  // dot = acc + off[a] + off[b] + 16384 ad
+      int32_t cb;
+      if (METRIC == L2) {  // d = na + nb - 2 dot
+        ra = nn - 2 * off - (U8 ? 2 * 16384 * (int32_t)a.ad : 0);
+        cb = nn - 2 * off;
+      } else {  // d = -dot
+        ra = -off - (U8 ? 16384 * (int32_t)a.ad : 0);
+        cb = -off;
       }
-      if (J + 1u < nb) {
-        if (J == I) {
-#pragma unroll
-          for (uint32_t st = 0; st < 4; st++) araw[st] = bnext[st];  // the next row block
-        }
-#pragma unroll
-        for (uint32_t st = 0; st < 4; st++) bcur[st] = bnext[st];
-      }
+      tb = prune_threshold(a.alpha, (double)Cd[j]) - cb;
     }
+    Nrm[j] = ra;
+    Tb[j] = tb;
+  }
+  wave_sync();
+  if (Mp > 0) {
+    const uint32_t nst = a.ad / 64u;
+    uint16_t* Pm16 = (uint16_t*)Pm;
+    if (nst == 1) gram_tiles<T, METRIC, 1>(a, Ci, Nrm, Tb, Pm16, Mp);
+    else if (nst == 2) gram_tiles<T, METRIC, 2>(a, Ci, Nrm, Tb, Pm16, Mp);
+    else if (nst == 3) gram_tiles<T, METRIC, 3>(a, Ci, Nrm, Tb, Pm16, Mp);
+    else gram_tiles<T, METRIC, 4>(a, Ci, Nrm, Tb, Pm16, Mp);
   }
   wave_sync();
 
-  // greedy sweep over the bit rows: lane w < kGramWords holds live word w
-  uint32_t live = 0;
+  // greedy sweep over the bit rows (index.h:84-106): lane w < kGramWords holds
+  // live word w; a step selects the lowest live position and drops every
+  // position its row marks.  Selections come in increasing position order:
+  // they are kept as a bit mask and written out in that order afterwards.
+  uint32_t live = 0, chosen = 0;
   if (lane < kGramWords) {
     const uint32_t lo = lane * 32u;
     live = Mp >= lo + 32u ? 0xFFFFFFFFu : (Mp > lo ? (1u << (Mp - lo)) - 1u : 0u);
@@ -417,15 +516,29 @@ __device__ __forceinline__ void prune_task_gram(const PruneArgs& a, uint32_t t, 
     const uint64_t nz = ballot(lane < kGramWords && live != 0u);
     if (!nz) break;
     const uint32_t w = (uint32_t)first_lane(nz);
-    const uint32_t lw = readlane_u32(live, (int)w);
-    const uint32_t at = w * 32u + (uint32_t)__builtin_ctz(lw);
-    if (lane == 0) sel[nsel] = Ci[at];
-    nsel++;
-    if (nsel >= a.Rsel) break;
+    const uint32_t bit = 1u << (uint32_t)__builtin_ctz(readlane_u32(live, (int)w));
+    const uint32_t at = w * 32u + (uint32_t)__builtin_ctz(bit);
+    if (lane == w) chosen |= bit;
+    if (++nsel >= a.Rsel) break;
     if (lane < kGramWords) {
       live &= ~Pm[at * kGramWords + lane];
-      if (lane == w) live &= ~(1u << (at & 31u));
+      if (lane == w) live &= ~bit;
     }
+  }
+  uint32_t cw[kGramWords], before[kGramWords];
+  uint32_t run = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kGramWords; w++) {
+    cw[w] = readlane_u32(chosen, (int)w);
+    before[w] = run;
+    run += popc(cw[w]);
+  }
+#pragma unroll
+  for (uint32_t u = 0; u < kSlots; u++) {
+    const uint32_t j = lane + u * kWave;
+    const uint32_t word = lane < 32u ? cw[2u * u] : cw[2u * u + 1u];
+    const uint32_t pre = lane < 32u ? before[2u * u] : before[2u * u + 1u];
+    if (j < Mp && ((word >> (j & 31u)) & 1u)) sel[pre + popc(word & ((1u << (j & 31u)) - 1u))] = Ci[j];
   }
   wave_sync();
   if (a.write_graph) {
@@ -493,7 +606,7 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
           atomicOr(a.status, 2u);
         }
       }
-    } else if (sizeof(T) == 1 && M64 <= a.gram_max) {
+    } else if (sizeof(T) == 1 && M64 <= a.gram_max && a.alpha > 0.0) {
       prune_task_gram<T, METRIC>(a, t, p, grow, a0, alen, gdeg, pv, sel, Cd, Ci, my + a.off_gram);
     } else if (M64 <= a.rrows) {
       prune_task<T, METRIC, true>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
