@@ -161,29 +161,33 @@ static uint32_t trav_stride_for(const ddg_index* idx, int trav) {
 
 // Fills the traversal fields of `a` and the query staging sizes; returns the
 // per-wave LDS bytes of the staged query (plus its FP8 factors).
+// float32 rows of more than one load batch whose reference order walks whole
+// groups of four (MIPS, or L2 with d % 4 == 0) are split over the fewest
+// lanes (a power of two) that leave each at most kSplitChunks chunks: one
+// round trip per row instead of one per batch of eight chunks (searches and
+// prunes).  DDG_NAT_SPLIT=0 keeps one lane per row (for measurement; the
+// results are the same).
+static void nat_split_for(const ddg_index* idx, uint32_t& S_out, uint32_t& cps_out) {
+  S_out = 0;
+  cps_out = 0;
+  const uint32_t chunks = idx->ad / 4;
+  const char* ns = getenv("DDG_NAT_SPLIT");
+  if (idx->dtype == DDG_F32 && chunks > kChunkBatch && (idx->metric == DDG_MIPS || idx->d % 4 == 0) &&
+      !(ns && ns[0] == '0')) {
+    uint32_t S = 2;
+    while (S < kWave && (chunks + S - 1) / S > kSplitChunks) S *= 2;
+    S_out = S;
+    cps_out = (chunks + S - 1) / S;
+  }
+}
+
 static uint32_t trav_layout(const ddg_index* idx, int trav, SearchArgs& a) {
   a.trav_split = 1;
   a.trav_cps = 0;
   if (trav == DDG_TRAV_NATIVE) {
     a.qrow_bytes = round_up(idx->ad * idx->es, 16);
     a.off_qaux = a.qrow_bytes;
-    // float32 rows of more than one load batch whose reference order walks
-    // whole groups of four (MIPS, or L2 with d % 4 == 0) are split over the
-    // fewest lanes (a power of two) that leave each at most kSplitChunks
-    // chunks: one round trip per row instead of one per batch of eight
-    // chunks.  DDG_NAT_SPLIT=0 keeps one lane per row (for measurement; the
-    // results are the same).
-    a.nat_split = 0;
-    a.nat_cps = 0;
-    const uint32_t chunks = idx->ad / 4;
-    const char* ns = getenv("DDG_NAT_SPLIT");
-    if (idx->dtype == DDG_F32 && chunks > kChunkBatch && (idx->metric == DDG_MIPS || idx->d % 4 == 0) &&
-        !(ns && ns[0] == '0')) {
-      uint32_t S = 2;
-      while (S < kWave && (chunks + S - 1) / S > kSplitChunks) S *= 2;
-      a.nat_split = S;
-      a.nat_cps = (chunks + S - 1) / S;
-    }
+    nat_split_for(idx, a.nat_split, a.nat_cps);
     return a.qrow_bytes;
   }
   const uint32_t esz = trav_esize(idx, trav);
@@ -691,6 +695,7 @@ static ddg_status prune_prepare(ddg_index* idx, PrunePlan& pp, uint64_t np, uint
   a.d = idx->d;
   a.ad = idx->ad;
   a.R1 = idx->R + 1;
+  nat_split_for(idx, a.nat_split, a.nat_cps);
   pp.l.blocks = (uint32_t)std::min<uint64_t>(pp.l.blocks, (np + pp.l.waves_per_block - 1) / pp.l.waves_per_block);
   // the queued tasks: one block of prune_big_waves() waves per task, 16
   // waves per CU (at most big_waves blocks: one slot of gcap entries each)

@@ -100,6 +100,9 @@ struct PruneArgs {
   // integer rows: tasks with at most gram_max candidates (<= 256) prune from
   // an MFMA Gram matrix; its norms, sums and prune bit matrix live at off_gram
   uint32_t gram_max, off_gram;
+  // float32 rows longer than one load batch: each distance split over
+  // nat_split lanes of nat_cps chunks (split_row_dist; 0 = one lane per row)
+  uint32_t nat_split, nat_cps;
 };
 
 struct Launch {

@@ -33,8 +33,12 @@
 namespace ddg {
 
 // One robustPrune task (index.h:63-110) with the wave: candidates, sort,
-// greedy selection, write-back.
-template <typename T, int METRIC, bool STAGED>
+// greedy selection, write-back.  SPLIT (float32 rows of more than one load
+// batch) spreads each distance over a group of a.nat_split lanes
+// (split_row_dist: all of the row's loads in flight at once, the same
+// additions in the same order); a pass then covers 64 / nat_split
+// candidates and the group's last lane stands for its candidate.
+template <typename T, int METRIC, bool STAGED, bool SPLIT = false>
 __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint32_t p, uint32_t* grow, uint64_t a0,
                                            uint64_t alen, uint32_t gdeg, T* pv, T* sv, uint32_t* sel, float* Cd,
                                            uint32_t* Ci, uint16_t* Cs, T* rows) {
@@ -45,6 +49,15 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
     const uint32_t nchunks = row_bytes >> 4;
     const T* base = (const T*)a.base;
     const uint32_t M = (uint32_t)(alen + gdeg);
+    const uint32_t lg = SPLIT ? (uint32_t)__builtin_ctz(a.nat_split) : 0u;
+    const uint32_t G = kWave >> lg;       // candidates per pass
+    const uint32_t li = lane >> lg;       // this lane's candidate in the pass
+    const uint32_t sub = lane & ((1u << lg) - 1u);
+    const bool rep = sub == (1u << lg) - 1u;  // the lane that holds the distance
+    auto row_dist = [&](const T* r, const T* q, const typename D::Q& qs) -> float {
+      if constexpr (SPLIT) return split_row_dist<METRIC>((const float*)r, (const float*)q, nchunks, a.nat_split, a.nat_cps, sub);
+      else return D::dist(r, q, qs, a.d, nchunks);
+    };
     stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
     // ---- candidates: A (given or computed distances) then G[p]
     for (uint32_t j = lane; j < M; j += kWave)
@@ -74,12 +87,18 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
       wave_sync();
     }
     const typename D::Q ps = D::prep(pv, nchunks);
-    for (uint32_t j = lane; j < M; j += kWave) {
-      const uint32_t id = Ci[j];
-      const T* r = STAGED ? rows + (uint64_t)j * a.rstride : base + (uint64_t)id * a.ad;
-      Cd[j] = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(r, pv, ps, a.d, nchunks);
-      if (STAGED) Cs[j] = (uint16_t)j;
+    // given distances first, then the computed ones (every lane of a pass
+    // takes part in a split distance; lanes past the list repeat its last row)
+    const uint32_t given = a.a_dists ? (uint32_t)alen : 0u;
+    for (uint32_t j = lane; j < given; j += kWave) Cd[j] = a.a_dists[a0 + j];
+    for (uint32_t b0 = given; b0 < M; b0 += G) {
+      const uint32_t j = b0 + li < M ? b0 + li : M - 1u;
+      const T* r = STAGED ? rows + (uint64_t)j * a.rstride : base + (uint64_t)Ci[j] * a.ad;
+      const float dj = row_dist(r, pv, ps);
+      if (rep && b0 + li < M) Cd[j] = dj;
     }
+    if (STAGED)
+      for (uint32_t j = lane; j < M; j += kWave) Cs[j] = (uint16_t)j;
     wave_sync();
 
     // ---- greedy selection (index.h:76-106) without a sort.  Sorting by
@@ -148,19 +167,16 @@ __device__ __forceinline__ void prune_task(const PruneArgs& a, uint32_t t, uint3
       bd = __builtin_inff();
       bi = bp = kPruned;
       uint32_t kept = 0;
-      for (uint32_t b0 = 0; b0 < live; b0 += kWave) {
-        const uint32_t j = b0 + lane;
-        bool keep = false;
-        uint32_t c = 0, cs = 0;
-        float cd = 0.f;
-        if (j < live && j != at) {
-          c = Ci[j];
-          cd = Cd[j];
-          if (STAGED) cs = Cs[j];
-          const T* r = STAGED ? rows + (uint64_t)cs * a.rstride : base + (uint64_t)c * a.ad;
-          const float dsp = D::dist(r, ps_row, ss, a.d, nchunks);
-          keep = !(a.alpha * (double)dsp <= (double)cd);
-        }
+      for (uint32_t b0 = 0; b0 < live; b0 += G) {
+        const uint32_t j = b0 + li;
+        const uint32_t jr = j < live ? j : live - 1u;  // a valid row for every lane of a split
+        const uint32_t c = Ci[jr];
+        const float cd = Cd[jr];
+        uint32_t cs = 0;
+        if (STAGED) cs = Cs[jr];
+        const T* r = STAGED ? rows + (uint64_t)cs * a.rstride : base + (uint64_t)c * a.ad;
+        const float dsp = row_dist(r, ps_row, ss);
+        const bool keep = rep && j < live && j != at && !(a.alpha * (double)dsp <= (double)cd);
         const uint64_t km = ballot(keep);
         wave_sync();  // every lane has read its entry (and p*'s row slot) before any is overwritten
         if (keep) {
@@ -610,6 +626,8 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
       prune_task_gram<T, METRIC>(a, t, p, grow, a0, alen, gdeg, pv, sel, Cd, Ci, my + a.off_gram);
     } else if (M64 <= a.rrows) {
       prune_task<T, METRIC, true>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
+    } else if (sizeof(T) == 4 && a.nat_split > 1u) {
+      if constexpr (sizeof(T) == 4) prune_task<T, METRIC, false, true>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
     } else {
       prune_task<T, METRIC, false>(a, t, p, grow, a0, alen, gdeg, pv, sv, sel, Cd, Ci, Cs, rows);
     }
