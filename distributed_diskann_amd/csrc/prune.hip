@@ -636,15 +636,34 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
 }
 
 // The queued (hub) tasks, one per workgroup: all waves of the block share a
-// task's candidate list in global scratch (a slot of gcap entries per block,
-// or the task's slice of the bump pool).  Each greedy step walks the live
-// list in rounds of one candidate per thread: distances to p*, the prune
-// test, then survivors compacted to the front (their order is free: a step
-// takes the live minimum by (distance, id), and entries with equal (distance,
-// id) are copies of one point), and the next minimum by a wave reduction and
-// one over the waves in LDS.  The same arithmetic as prune_task, so the
-// selections are identical; the block only spreads each step's row reads
-// over more lanes.
+// task.  Its candidates and their distances to p go to global scratch (a
+// slot of gcap entries per block, or the task's slice of the bump pool), and
+// the greedy sweep (index.h:76-106) then runs over windows of the list in
+// (distance, id) order: a window is every remaining candidate whose distance
+// key is at most a bound chosen so that at most kWin of them qualify (a
+// radix select over the keys), gathered into LDS.  Inside a window the sweep
+// is the plain one: take the live minimum by (distance, id), prune the
+// window's other live candidates against it, compact the survivors.  Every
+// candidate left in the list comes after the whole window in sorted order,
+// so when a window runs out before Rsel selections the rest of the list is
+// pruned against the window's selections and compacted, and the next window
+// is taken from what remains.  A hub list of 10^5 candidates is swept over
+// its first few thousand only, instead of every greedy step reading all of
+// it.  A list of at most kWin candidates, and one where a single distance
+// value has more than kWin candidates, is swept in place in scratch.  One
+// thread per candidate row (spreading float32 rows over lanes measured
+// slower here: a step's rounds multiply with the lanes per row).  The same
+// arithmetic as prune_task, so the selections are identical.
+constexpr uint32_t kWin = 2048, kHistBins = 2048;
+
+// The ordered key of a distance: the bits sign-folded (zero taken as +0.0),
+// so the unsigned order of keys is the float order.
+__device__ __forceinline__ uint32_t dist_key(float d) {
+  uint32_t b = __float_as_uint(d);
+  if ((b << 1) == 0u) b = 0u;
+  return b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u);
+}
+
 template <typename T, int METRIC>
 __global__ __launch_bounds__(1024) void k_prune_big(PruneArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -662,7 +681,10 @@ __global__ __launch_bounds__(1024) void k_prune_big(PruneArgs a) {
   float* wd = (float*)(wcnt + kMaxBigWaves);
   uint32_t* wi = (uint32_t*)(wd + kMaxBigWaves);
   uint32_t* wp = wi + kMaxBigWaves;
-  uint32_t* task = wp + kMaxBigWaves;
+  uint32_t* task = wp + kMaxBigWaves;   // 4 words: task, selected bin, residual rank
+  float* Wd = (float*)(task + 4);       // the window (LDS)
+  uint32_t* Wi = (uint32_t*)(Wd + kWin);
+  uint32_t* hist = Wi + kWin;
   const T* base = (const T*)a.base;
   const uint64_t ntasks = a.nbig[0];
 
@@ -685,10 +707,11 @@ __global__ __launch_bounds__(1024) void k_prune_big(PruneArgs a) {
     bi = bcast0(bi);
     __syncthreads();  // the slots are read before any later write
   };
-  // one compaction round: thread tid keeps (c, cd) when keep; survivors go to
-  // Ci / Cd from position `out` on; returns the round's survivors.  Every
-  // entry of the round was read before the barrier, and a survivor's new
-  // position is at most its old one, so no unread entry is overwritten.
+  // one compaction round: threads with keep hold (c, cd); survivors go to
+  // Ci / Cd from position `out` on, in thread order; returns the round's
+  // survivors.  Every entry of the round was read before the barrier, and a
+  // survivor's new position is at most its old one (in place) or the
+  // destination is another array, so no unread entry is overwritten.
   auto compact = [&](float* Cd, uint32_t* Ci, bool keep, uint32_t c, float cd, uint32_t out) {
     const uint64_t km = ballot(keep);
     if (lane == 0) wcnt[wave] = popc(km);
@@ -707,6 +730,25 @@ __global__ __launch_bounds__(1024) void k_prune_big(PruneArgs a) {
     }
     __syncthreads();  // stores and counts done before the next round
     return total;
+  };
+  // exclusive prefix of v over the block's threads; total on every thread
+  auto block_scan = [&](uint32_t v, uint32_t& total) {
+    uint32_t x = v;
+    for (int o = 1; o < (int)kWave; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if ((int)lane >= o) x += y;
+    }
+    if (lane == kWave - 1u) wcnt[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (uint32_t w = 0; w < W; w++) {
+      const uint32_t n = wcnt[w];
+      before += w < wave ? n : 0u;
+      total += n;
+    }
+    __syncthreads();
+    return before + x - v;
   };
 
   for (;;) {
@@ -729,17 +771,11 @@ __global__ __launch_bounds__(1024) void k_prune_big(PruneArgs a) {
     const uint32_t gdeg = a.add ? deg : 0u;
     const uint32_t M = (uint32_t)(alen + gdeg);
 
-    // ---- candidates: A (given or computed distances) then G[p]
+    // ---- candidates: A (given or computed distances) then G[p]; copies of
+    // p dropped
     if (wave == 0) stage_row(pv, base + (uint64_t)p * a.ad, row_bytes);
-    for (uint32_t j = tid; j < M; j += nth) Ci[j] = j < alen ? a.a_ids[a0 + j] : grow[1 + (j - (uint32_t)alen)];
     __syncthreads();
     const typename D::Q ps = D::prep(pv, nchunks);
-    for (uint32_t j = tid; j < M; j += nth)
-      Cd[j] = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(base + (uint64_t)Ci[j] * a.ad, pv, ps, a.d, nchunks);
-    __syncthreads();
-
-    // ---- greedy selection (index.h:76-106): drop copies of p, then take the
-    // live minimum and prune against it, step by step
     bd = __builtin_inff();
     bi = bp = kPruned;
     uint32_t live = 0;
@@ -747,38 +783,152 @@ __global__ __launch_bounds__(1024) void k_prune_big(PruneArgs a) {
       const uint32_t j = b0 + tid;
       uint32_t c = 0;
       float cd = 0.f;
-      if (j < M) { c = Ci[j]; cd = Cd[j]; }
+      if (j < M) {
+        c = j < alen ? a.a_ids[a0 + j] : grow[1 + (j - (uint32_t)alen)];
+        cd = (j < alen && a.a_dists) ? a.a_dists[a0 + j] : D::dist(base + (uint64_t)c * a.ad, pv, ps, a.d, nchunks);
+      }
       live += compact(Cd, Ci, j < M && c != p, c, cd, live);
     }
-    block_min();
 
     uint32_t nsel = 0;
-    while (bp != kPruned) {
-      const uint32_t pstar = bi, at = bp;
-      if (tid == 0) sel[nsel] = pstar;
-      nsel++;
-      if (nsel >= a.Rsel) break;  // the reference's remaining prunes cannot change the result
-      if (wave == 0) stage_row(sv, base + (uint64_t)pstar * a.ad, row_bytes);
-      __syncthreads();
-      const typename D::Q ss = D::prep(sv, nchunks);
-      bd = __builtin_inff();
-      bi = bp = kPruned;
-      uint32_t kept = 0;
-      for (uint32_t b0 = 0; b0 < live; b0 += nth) {
-        const uint32_t j = b0 + tid;
-        bool keep = false;
-        uint32_t c = 0;
-        float cd = 0.f;
-        if (j < live && j != at) {
-          c = Ci[j];
-          cd = Cd[j];
-          const float dsp = D::dist(base + (uint64_t)c * a.ad, sv, ss, a.d, nchunks);
-          keep = !(a.alpha * (double)dsp <= (double)cd);
+    for (bool done = false; !done;) {
+      // ---- the next window's bound: one below the (kWin + 1)-th smallest
+      // remaining key (a radix select over 11 + 11 + 10 bits); at most kWin
+      // remaining are swept in place
+      uint32_t tau = 0xFFFFFFFFu, rank = kWin + 1u, prefix = 0, pmask = 0;
+      bool last = live <= kWin, in_place = last;
+      for (uint32_t pass = 0; pass < 3 && !last; pass++) {
+        const uint32_t shift = pass == 0 ? 21u : (pass == 1 ? 10u : 0u);
+        const uint32_t nbins = pass == 2 ? 1024u : 2048u;
+        for (uint32_t i = tid; i < nbins; i += nth) hist[i] = 0u;
+        __syncthreads();
+        for (uint32_t j = tid; j < live; j += nth) {
+          const uint32_t k = dist_key(Cd[j]);
+          if ((k & pmask) == prefix) atomicAdd(&hist[(k >> shift) & (nbins - 1u)], 1u);
         }
-        kept += compact(Cd, Ci, keep, c, cd, kept);
+        __syncthreads();
+        const uint32_t per = nbins / nth ? nbins / nth : 1u;  // bins per thread
+        const uint32_t b0 = tid * per;
+        uint32_t own = 0;
+        for (uint32_t i = 0; i < per; i++) own += b0 + i < nbins ? hist[b0 + i] : 0u;
+        uint32_t total = 0;
+        const uint32_t excl = block_scan(own, total);
+        if (excl < rank && rank <= excl + own) {  // this thread's bins hold the rank-th key
+          uint32_t cum = excl, b = b0;
+          while (cum + hist[b] < rank) cum += hist[b++];
+          task[1] = b;
+          task[2] = rank - cum;
+        }
+        __syncthreads();
+        prefix |= bcast0(task[1]) << shift;
+        pmask |= (nbins - 1u) << shift;
+        rank = bcast0(task[2]);
+        __syncthreads();
       }
-      live = kept;
-      block_min();  // its barriers also order the reads of sv before the next stage
+      if (!last) {
+        // kWin + 1 - rank remaining keys lie below the selected one
+        if (rank == kWin + 1u) in_place = last = true;  // ties beyond kWin: sweep the rest in place
+        else tau = prefix - 1u;
+      }
+      // ---- gather the window (in place: the list as it stands, whose last
+      // compaction offered every entry)
+      float* Gd = in_place ? Cd : Wd;
+      uint32_t* Gi = in_place ? Ci : Wi;
+      uint32_t wl = live;
+      if (!in_place) {
+        bd = __builtin_inff();
+        bi = bp = kPruned;
+        wl = 0;
+        for (uint32_t b0 = 0; b0 < live; b0 += nth) {
+          const uint32_t j = b0 + tid;
+          bool keep = false;
+          uint32_t c = 0;
+          float cd = 0.f;
+          if (j < live) {
+            c = Ci[j];
+            cd = Cd[j];
+            keep = dist_key(cd) <= tau;
+          }
+          wl += compact(Gd, Gi, keep, c, cd, wl);
+        }
+      }
+      block_min();
+      const uint32_t sel0 = nsel;
+
+      // ---- the sweep inside the window
+      while (bp != kPruned) {
+        const uint32_t pstar = bi, at = bp;
+        if (tid == 0) sel[nsel] = pstar;
+        nsel++;
+        if (nsel >= a.Rsel) break;  // the reference's remaining prunes cannot change the result
+        if (wave == 0) stage_row(sv, base + (uint64_t)pstar * a.ad, row_bytes);
+        __syncthreads();
+        const typename D::Q ss = D::prep(sv, nchunks);
+        bd = __builtin_inff();
+        bi = bp = kPruned;
+        uint32_t kept = 0;
+        for (uint32_t b0 = 0; b0 < wl; b0 += nth) {
+          const uint32_t j = b0 + tid;
+          bool keep = false;
+          uint32_t c = 0;
+          float cd = 0.f;
+          if (j < wl && j != at) {
+            c = Gi[j];
+            cd = Gd[j];
+            const float dsp = D::dist(base + (uint64_t)c * a.ad, sv, ss, a.d, nchunks);
+            keep = !(a.alpha * (double)dsp <= (double)cd);
+          }
+          kept += compact(Gd, Gi, keep, c, cd, kept);
+        }
+        wl = kept;
+        block_min();  // its barriers also order the reads of sv before the next stage
+      }
+      done = last || nsel >= a.Rsel;
+      if (!done) {
+        // ---- the rest of the list (keys above the window's bound) pruned
+        // against the window's selections, one compacting pass each, so a
+        // candidate pruned early is not tested again
+        __syncthreads();  // sel written
+        bd = __builtin_inff();
+        bi = bp = kPruned;
+        uint32_t kept = 0;
+        for (uint32_t b0 = 0; b0 < live; b0 += nth) {  // the window's entries leave
+          const uint32_t j = b0 + tid;
+          bool keep = false;
+          uint32_t c = 0;
+          float cd = 0.f;
+          if (j < live) {
+            c = Ci[j];
+            cd = Cd[j];
+            keep = dist_key(cd) > tau;
+          }
+          kept += compact(Cd, Ci, keep, c, cd, kept);
+        }
+        live = kept;
+        for (uint32_t i = sel0; i < nsel && live > 0; i++) {
+          if (wave == 0) stage_row(sv, base + (uint64_t)sel[i] * a.ad, row_bytes);
+          __syncthreads();
+          const typename D::Q ss = D::prep(sv, nchunks);
+          bd = __builtin_inff();
+          bi = bp = kPruned;
+          kept = 0;
+          for (uint32_t b0 = 0; b0 < live; b0 += nth) {
+            const uint32_t j = b0 + tid;
+            bool keep = false;
+            uint32_t c = 0;
+            float cd = 0.f;
+            if (j < live) {
+              c = Ci[j];
+              cd = Cd[j];
+              const float dsc = D::dist(base + (uint64_t)c * a.ad, sv, ss, a.d, nchunks);
+              keep = !(a.alpha * (double)dsc <= (double)cd);
+            }
+            kept += compact(Cd, Ci, keep, c, cd, kept);
+          }
+          live = kept;
+        }
+      }
+      __syncthreads();  // sel and the window are read before the next gather writes
     }
     __syncthreads();
     if (a.write_graph) {
@@ -794,7 +944,8 @@ __global__ __launch_bounds__(1024) void k_prune_big(PruneArgs a) {
 }
 
 uint32_t prune_big_lds_bytes(uint32_t row_bytes, uint32_t Rsel) {
-  return 2u * ((row_bytes + 15u) & ~15u) + ((Rsel * 4u + 15u) & ~15u) + 4u * kMaxBigWaves * 4u + 16u;
+  return 2u * ((row_bytes + 15u) & ~15u) + ((Rsel * 4u + 15u) & ~15u) + 4u * kMaxBigWaves * 4u + 16u +
+         8u * kWin + 4u * kHistBins;
 }
 
 template <typename T, int M>

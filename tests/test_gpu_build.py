@@ -69,6 +69,49 @@ def test_build_hub_identical(oracle):
     assert np.array_equal(canon(rows), canon(G.rows))
 
 
+def test_build_hub_windows(oracle):
+    """Tight clusters at staggered distances around a centre point (cluster k
+    at radius 1 + 0.02 k).  Every point links to the centre, so its reverse
+    lists hold thousands of candidates ordered cluster by cluster, and each
+    selection prunes only its own cluster: the hub sweep needs more than one
+    window of the nearest candidates, and a later window's candidates from a
+    cluster that straddles the window bound are pruned by a point selected in
+    an earlier window."""
+    print("[gpu] build clustered star (windowed hub sweep) 18001 x 16", flush=True)
+    from distributed_diskann_amd.formats import align_rows
+    rng = np.random.default_rng(33)
+    K, per, d = 60, 300, 16
+    dirs = rng.standard_normal((K, d))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    cent = dirs * (1.0 + 0.02 * np.arange(K))[:, None]
+    X = np.concatenate([np.zeros((1, d)), np.repeat(cent, per, 0) + 0.01 * rng.standard_normal((K * per, d))])
+    X = X[np.concatenate([[0], 1 + rng.permutation(K * per)])].astype(np.float32)
+    Xa = align_rows(X)
+    G = oracle.build(oracle.Points(Xa, d), 16, 32, 1.2, tie_mode=oracle.TIE_ID)
+    idx, rows, secs = _gpu_build(Xa, d, "l2", 16, L=32, alpha=1.2)
+    assert np.array_equal(canon(rows), canon(G.rows))
+
+
+def test_build_hub_ties(oracle):
+    """A centre point (the zero row, id 0) and 0/1 rows with exactly four
+    ones in 64 columns: the centre is at distance 4 from every point and
+    prunes its other candidates, so every point links to it, and its reverse
+    lists hold thousands of candidates at one distance.  More than a window's
+    worth of equal keys: the hub sweep runs over the list in place, in id
+    order."""
+    print("[gpu] build tied star (hub list of equal distances) 12001 x 64 uint8", flush=True)
+    from distributed_diskann_amd.formats import align_rows
+    rng = np.random.default_rng(34)
+    n, d = 12000, 64
+    X = np.zeros((n + 1, d), np.uint8)
+    for i in range(1, n + 1):
+        X[i, rng.choice(d, 4, replace=False)] = 1
+    Xa = align_rows(X)
+    G = oracle.build(oracle.Points(Xa, d), 16, 32, 1.2, tie_mode=oracle.TIE_ID)
+    idx, rows, secs = _gpu_build(Xa, d, "l2", 16, L=32, alpha=1.2)
+    assert np.array_equal(canon(rows), canon(G.rows))
+
+
 def test_build_n0_identical_and_recall(oracle, n0):
     print("[gpu] build n0 (10k x 20, R32 L50)", flush=True)
     from distributed_diskann_amd.index import recall_with_ties
