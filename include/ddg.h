@@ -224,7 +224,9 @@ ddg_status ddg_prepare_traversal(ddg_index* idx, ddg_trav trav, double* seconds)
  * waves per block, 1..8 resident blocks per CU -- on these queries
  * (nq*aligned_d, host memory) with qp, checks that every shape returns the
  * same ids, and keeps the fastest for this handle's later searches with the
- * same traversal mode and filter size.  The filter size and frontier width
+ * same traversal mode and filter size (each shape timed over one launch
+ * after a warm-up; the three fastest and the default again over five, the
+ * smallest median winning).  The filter size and frontier width
  * are fixed by beamSearch.h:64-73 and the beam, so they are not tuned.
  * For rows longer than 128 bytes it also tries gathering 2, 4, 8 or as
  * many candidate rows as one load batch holds into LDS per round trip.
