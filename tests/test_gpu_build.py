@@ -33,13 +33,15 @@ def test_build_identical_to_oracle(oracle, dtype, metric, d, alpha):
 @pytest.mark.parametrize("dtype,metric,d,hi,alpha", [("uint8", "l2", 128, 3, 1.2), ("uint8", "l2", 64, 2, 1.0),
                                                      ("int8", "mips", 32, 2, 1.0), ("int8", "l2", 256, 128, 1.5),
                                                      ("uint8", "l2", 160, 4, 1.2), ("uint8", "mips", 200, 3, 1.0),
-                                                     ("int8", "l2", 100, 3, 0.95), ("uint8", "l2", 256, 256, 1.2)])
+                                                     ("int8", "l2", 100, 3, 0.95), ("uint8", "l2", 256, 256, 1.2),
+                                                     ("uint8", "l2", 64, 3, 0.0)])
 def test_build_integer_ties(oracle, dtype, metric, d, hi, alpha):
     """Low-entropy integer rows: many equal distances, and prune tests that
     land exactly on alpha * d(a, b) == d(p, b) (the matrix-core prune decides
     them by integer thresholds).  Row widths 64, 128, 192 and 256 bytes (one
     to four K steps per tile); the last case reaches the largest uint8
-    distances (255^2 x 256, just below 2^24)."""
+    distances (255^2 x 256, just below 2^24); alpha = 0 takes the pairwise
+    prune instead of the matrix-core one (every test is 0 <= d(p, b))."""
     print(f"[gpu] build integer ties {dtype} {metric} d={d} values<{hi} alpha={alpha}", flush=True)
     from distributed_diskann_amd.formats import align_rows
     rng = np.random.default_rng(d + hi)
