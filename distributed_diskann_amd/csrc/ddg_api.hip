@@ -635,14 +635,14 @@ static ddg_status run_insert_search(ddg_index* idx, const uint32_t* pts, uint64_
 }
 
 // Long-list scratch a prune can need: a task takes the global path only when
-// its M = |A| + deg candidates exceed ccap, and takes pow2(M) < 2M entries.
-// With `total` candidates over np tasks and deg <= Radd, the sum is below
-// 2 (total + np Radd), and below 2 total (1 + Radd / (ccap - Radd)) because
-// each such task has |A| > ccap - Radd.
+// its M = |A| + deg candidates exceed ccap, and takes M entries.  With
+// `total` candidates over np tasks and deg <= Radd, the sum is at most
+// total + np Radd, and below total (1 + Radd / (ccap - Radd)) because each
+// such task has |A| > ccap - Radd.
 static uint64_t prune_pool(uint64_t total, uint64_t np, uint32_t Radd, uint32_t ccap, uint32_t max_cand) {
   if (max_cand <= ccap) return 0;
-  uint64_t b = 2 * (total + np * (uint64_t)Radd);
-  if (ccap > Radd) b = std::min<uint64_t>(b, 2 * total + (2 * total * Radd + (ccap - Radd) - 1) / (ccap - Radd));
+  uint64_t b = total + np * (uint64_t)Radd;
+  if (ccap > Radd) b = std::min<uint64_t>(b, total + (total * Radd + (ccap - Radd) - 1) / (ccap - Radd));
   return b + 64;
 }
 

@@ -605,8 +605,7 @@ __global__ __launch_bounds__(256) void k_prune(PruneArgs a) {
       if (lane == 0) {
         const uint32_t s = atomicAdd(a.nbig, 1u);
         if (a.big_off) {
-          uint64_t need = 1;
-          while (need < M64) need <<= 1;  // the bitonic sort pads to a power of two
+          const uint64_t need = M64;  // the task's candidates, nothing more
           const uint64_t off = atomicAdd((unsigned long long*)a.gused, (unsigned long long)need);
           if (off + need <= a.gpool) {
             a.big_tasks[s] = t;
