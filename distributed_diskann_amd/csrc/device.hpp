@@ -319,7 +319,10 @@ __device__ __forceinline__ float split_row_dist(const float* row, const float* q
     xs[u] = r[(u < cps && c < ng) ? c : 0u];
   }
   // the terms this lane adds: four products per chunk (MIPS), or one sum of
-  // squares per chunk (L2: ((e0^2 + e1^2) + e2^2) + e3^2, NSGDist.h:113-131)
+  // squares per chunk (L2: ((e0^2 + e1^2) + e2^2) + e3^2, NSGDist.h:113-131).
+  // A slot past the lane's chunks holds +0.0 terms: the running sum starts
+  // at +0.0 and never becomes -0.0 (an exact zero sum is +0.0), so adding
+  // +0.0 leaves its bits as they are, and the chain below needs no selects.
   float t[kSplitChunks][4];
   bool ok[kSplitChunks];
 #pragma unroll
@@ -335,12 +338,12 @@ __device__ __forceinline__ float split_row_dist(const float* row, const float* q
       gs = gs + e1 * e1;
       gs = gs + e2 * e2;
       gs = gs + e3 * e3;
-      t[u][0] = gs;
+      t[u][0] = ok[u] ? gs : 0.f;
     } else {
-      t[u][0] = b.x * a.x;
-      t[u][1] = b.y * a.y;
-      t[u][2] = b.z * a.z;
-      t[u][3] = b.w * a.w;
+      t[u][0] = ok[u] ? b.x * a.x : 0.f;
+      t[u][1] = ok[u] ? b.y * a.y : 0.f;
+      t[u][2] = ok[u] ? b.z * a.z : 0.f;
+      t[u][3] = ok[u] ? b.w * a.w : 0.f;
     }
   }
   float acc = 0.f;
@@ -350,16 +353,14 @@ __device__ __forceinline__ float split_row_dist(const float* row, const float* q
       float v = s == 0 ? 0.f : in;
 #pragma unroll
       for (uint32_t u = 0; u < kSplitChunks; u++) {
-        float w = v;
         if (METRIC == L2) {
-          w = w + t[u][0];
+          v = v + t[u][0];
         } else {
-          w = w + t[u][0];
-          w = w + t[u][1];
-          w = w + t[u][2];
-          w = w + t[u][3];
+          v = v + t[u][0];
+          v = v + t[u][1];
+          v = v + t[u][2];
+          v = v + t[u][3];
         }
-        v = ok[u] ? w : v;
       }
       acc = v;
     }
