@@ -176,8 +176,11 @@ static void nat_split_for(const ddg_index* idx, uint32_t& S_out, uint32_t& cps_o
       !(ns && ns[0] == '0')) {
     uint32_t S = 2;
     while (S < kWave && (chunks + S - 1) / S > kSplitChunks) S *= 2;
-    S_out = S;
-    cps_out = (chunks + S - 1) / S;
+    // rows beyond 64 lanes of kSplitChunks (d > 2048) keep one lane per row
+    if ((chunks + S - 1) / S <= kSplitChunks) {
+      S_out = S;
+      cps_out = (chunks + S - 1) / S;
+    }
   }
 }
 

@@ -303,18 +303,23 @@ struct Dist<float, METRIC> {
 // adds are the one-lane walk's, in its order, so the bits are the same.  All
 // lanes of the wave call it with the same S / cps; every lane's row pointer
 // must be valid (lanes without a row pass any valid row, their result is
-// discarded).  The last lane of each group returns the distance.
+// discarded).  The last lane of each group returns the distance.  MAXC is
+// the number of chunk slots a lane walks (cps <= MAXC): instantiated for the
+// share itself, the chain adds nothing for slots past it.
 constexpr uint32_t kSplitChunks = 8;  // at most this many chunks per lane
+// float32 rows are whole 64-byte multiples (4 chunks), so a split over the
+// fewest lanes leaving at most kSplitChunks gives shares of 5 to 8 chunks
+constexpr uint32_t kSplitChunksMin = 5;
 
-template <int METRIC>
+template <int METRIC, uint32_t MAXC = kSplitChunks>
 __device__ __forceinline__ float split_row_dist(const float* row, const float* q, uint32_t ng, uint32_t S,
                                                 uint32_t cps, uint32_t sub) {
   const float4* q4 = (const float4*)q;
   const uint4* r = (const uint4*)row;
   const uint32_t cb = sub * cps;
-  uint4 xs[kSplitChunks];
+  uint4 xs[MAXC];
 #pragma unroll
-  for (uint32_t u = 0; u < kSplitChunks; u++) {
+  for (uint32_t u = 0; u < MAXC; u++) {
     const uint32_t c = cb + u;
     xs[u] = r[(u < cps && c < ng) ? c : 0u];
   }
@@ -323,10 +328,10 @@ __device__ __forceinline__ float split_row_dist(const float* row, const float* q
   // A slot past the lane's chunks holds +0.0 terms: the running sum starts
   // at +0.0 and never becomes -0.0 (an exact zero sum is +0.0), so adding
   // +0.0 leaves its bits as they are, and the chain below needs no selects.
-  float t[kSplitChunks][4];
-  bool ok[kSplitChunks];
+  float t[MAXC][4];
+  bool ok[MAXC];
 #pragma unroll
-  for (uint32_t u = 0; u < kSplitChunks; u++) {
+  for (uint32_t u = 0; u < MAXC; u++) {
     const uint32_t c = cb + u;
     ok[u] = u < cps && c < ng;
     const float4 b = q4[ok[u] ? c : 0u];
@@ -352,7 +357,7 @@ __device__ __forceinline__ float split_row_dist(const float* row, const float* q
     if (sub == s) {
       float v = s == 0 ? 0.f : in;
 #pragma unroll
-      for (uint32_t u = 0; u < kSplitChunks; u++) {
+      for (uint32_t u = 0; u < MAXC; u++) {
         if (METRIC == L2) {
           v = v + t[u][0];
         } else {

@@ -38,6 +38,11 @@ namespace ddg {
 
 constexpr uint32_t kBuckets = 1024;  // slot buckets for the lane-conflict check
 constexpr int kRowsLane = 0, kRowsStage = 1, kRowsSplit = 2;
+// split rows whose lanes hold 5, 6 or 7 chunks (kRowsSplit: 8), each its own
+// instantiation so the sum's chain walks exactly the lane's share
+constexpr int kRowsSplit5 = 5, kRowsSplit6 = 6, kRowsSplit7 = 7;
+constexpr bool split_rows(int R) { return R == kRowsSplit || R >= kRowsSplit5; }
+constexpr uint32_t split_share(int R) { return R == kRowsSplit ? kSplitChunks : (uint32_t)R; }
 
 // Frontier and candidate entries are (distance bits, id) pairs; bit 31 of a
 // frontier id marks a visited entry.  Order: (distance, id) of
@@ -118,7 +123,7 @@ __device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t 
 // rows split over a.nat_split lanes, split_row_dist); each is its own
 // instantiation, so no kernel carries another's registers.
 template <typename T, int METRIC, int TRAV, bool GTAB, int ROWS>
-__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? (ROWS == kRowsSplit ? 4 : 5) : (TRAV == TRAV_BF16 ? 5 : 1)))
+__global__ __launch_bounds__(256, (TRAV == TRAV_NATIVE ? (split_rows(ROWS) ? 4 : 5) : (TRAV == TRAV_BF16 ? 5 : 1)))
 void k_search(SearchArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   using P = TravDist<T, METRIC, TRAV>;
@@ -445,7 +450,7 @@ void k_search(SearchArgs a) {
         // pending load at a loop header, which would stall on the prefetch
         if (nk > 0) batch(0);
         for (uint32_t b0 = step; b0 < nk; b0 += step) batch(b0);
-      } else if constexpr (ROWS == kRowsSplit) {
+      } else if constexpr (split_rows(ROWS)) {
         // long float32 rows: one row over nat_split lanes, all its chunks in
         // flight at once, the running sum handed along the lanes in the
         // reference's order (split_row_dist)
@@ -456,8 +461,8 @@ void k_search(SearchArgs a) {
           const uint32_t j = b0 + lane / S;
           const bool v = j < nk;
           const uint32_t id = Kid[v ? j : b0];  // lanes past the list read a row of the batch
-          const float dd = split_row_dist<METRIC>((const float*)a.base + (uint64_t)id * a.ad, (const float*)ql,
-                                                  a.ad >> 2, S, a.nat_cps, sub);
+          const float dd = split_row_dist<METRIC, split_share(ROWS)>((const float*)a.base + (uint64_t)id * a.ad,
+                                                                     (const float*)ql, a.ad >> 2, S, a.nat_cps, sub);
           const bool pass = v && sub == S - 1u && !(dd >= cutoff);
           const uint64_t pm = ballot(pass);
           if (pass) Cn[nc + mbcnt(pm)] = cand_key(dd, id);
@@ -809,8 +814,14 @@ static hipError_t launch_t(const SearchArgs& a, const Launch& l, hipStream_t s) 
   if (a.stage_rows)
     return a.gtable ? launch_g<T, M, TR, true, kRowsStage>(a, l, s) : launch_g<T, M, TR, false, kRowsStage>(a, l, s);
   if constexpr (has_split<T, TR>()) {
-    if (a.nat_split > 1u)
-      return a.gtable ? launch_g<T, M, TR, true, kRowsSplit>(a, l, s) : launch_g<T, M, TR, false, kRowsSplit>(a, l, s);
+    if (a.nat_split > 1u) {
+      switch (a.nat_cps) {
+        case 5: return a.gtable ? launch_g<T, M, TR, true, kRowsSplit5>(a, l, s) : launch_g<T, M, TR, false, kRowsSplit5>(a, l, s);
+        case 6: return a.gtable ? launch_g<T, M, TR, true, kRowsSplit6>(a, l, s) : launch_g<T, M, TR, false, kRowsSplit6>(a, l, s);
+        case 7: return a.gtable ? launch_g<T, M, TR, true, kRowsSplit7>(a, l, s) : launch_g<T, M, TR, false, kRowsSplit7>(a, l, s);
+        default: return a.gtable ? launch_g<T, M, TR, true, kRowsSplit>(a, l, s) : launch_g<T, M, TR, false, kRowsSplit>(a, l, s);
+      }
+    }
   }
   return a.gtable ? launch_g<T, M, TR, true, kRowsLane>(a, l, s) : launch_g<T, M, TR, false, kRowsLane>(a, l, s);
 }
@@ -833,6 +844,12 @@ static void set_attrs_t(int lds) {
   if constexpr (has_split<T, TR>()) {
     (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, kRowsSplit>, at, lds);
     (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, kRowsSplit>, at, lds);
+    (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, kRowsSplit5>, at, lds);
+    (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, kRowsSplit5>, at, lds);
+    (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, kRowsSplit6>, at, lds);
+    (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, kRowsSplit6>, at, lds);
+    (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, true, kRowsSplit7>, at, lds);
+    (void)hipFuncSetAttribute((const void*)&k_search<T, M, TR, false, kRowsSplit7>, at, lds);
   }
   (void)hipFuncSetAttribute((const void*)&k_trav_dist<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
   (void)hipFuncSetAttribute((const void*)&k_start_keys<T, M, TR>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);

@@ -1,7 +1,9 @@
 """Native float32 rows split over lanes (GPU).
 
 A float32 row of more than eight 16-byte chunks is spread over S lanes (2, 4,
-8 or 16: the fewest leaving a lane at most 8 chunks), which load their
+8, 16, ...: the fewest leaving a lane at most 8 chunks, each instantiation
+walking exactly the lane's 5-8 chunks; rows of more than 64 x 8 chunks keep
+one lane per row), which load their
 chunks together and hand the running FP32 sum along in the reference's
 order (euclidian_point.h / NSGDist.h:113-131 for L2, mips_point.h:60-66 for
 MIPS).  Builds must equal the oracle's row for row and searches must be
@@ -16,8 +18,9 @@ from conftest import canon
 
 pytestmark = pytest.mark.gpu
 
-# (metric, d): aligned row chunks 12 (S=2), 28 (S=4), 52 (S=8), 76 (S=16); d = 37 (L2) keeps one lane per row
-CASES = [("l2", 40), ("mips", 100), ("l2", 200), ("mips", 300), ("l2", 300), ("l2", 37)]
+# (metric, d): aligned row chunks 12 (S=2), 28 (S=4), 52 (S=8), 76 (S=16); d = 37 (L2) keeps one lane per row,
+# and so does d = 2100 (528 chunks: more than 64 lanes of 8)
+CASES = [("l2", 40), ("mips", 100), ("l2", 200), ("mips", 300), ("l2", 300), ("l2", 37), ("mips", 2100)]
 
 
 def _data(d, n=3000, nq=200, seed=3):
