@@ -267,12 +267,13 @@ def cpu_baseline(X, Q, w, G, gi, gd, Ls, build_n):
     search_s = time.perf_counter() - t0
     rec = recall_with_ties(ci, gi, gd, 10)
     log(f"[bench] CPU oracle ({threads} threads, {info['model']}): build of {m} points {build_s:.2f} s, "
-        f"FP32 search {Q.shape[0] / search_s:.0f} QPS at Ls={Ls} (recall {rec:.4f})")
+        f"search {Q.shape[0] / search_s:.0f} QPS at Ls={Ls} (recall {rec:.4f})")
     return {"value": Q.shape[0] / search_s, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"FP32 search of all {Q.shape[0]} queries at Ls={Ls} over the GPU-built "
-                      f"{X.shape[0]}-point graph (recall {rec:.4f}); build of the first {m} points in "
-                      f"{build_s:.2f} s ({m / build_s:.0f} points/s); the full {X.shape[0]}-point CPU build "
-                      "does not fit this run and is not extrapolated",
+            "sample": f"search of all {Q.shape[0]} queries at Ls={Ls} over the GPU-built "
+                      f"{X.shape[0]}-point graph (recall {rec:.4f}); " +
+                      (f"build of all {m} points in {build_s:.2f} s" if m == X.shape[0] else
+                       f"build of the first {m} points in {build_s:.2f} s ({m / build_s:.0f} points/s); the full "
+                       f"{X.shape[0]}-point CPU build does not fit this run and is not extrapolated"),
             "search_seconds": search_s, "build_points_sample": m, "build_seconds_sample": build_s,
             "cpu": info}
 
@@ -289,6 +290,9 @@ def main():
                     help="headline workload (mips: the 10M x 200 float32 set; the integer and 100K sets search "
                          "in their native arithmetic only; used for per-workload profiles)")
     ap.add_argument("--cpu-build-n", type=int, default=200_000)
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="also time the CPU oracle on the smaller configurations (and run the 10K x 20 one): "
+                         "their sub-blocks then carry a cpu_baseline")
     ap.add_argument("--only-gt", action="store_true",
                     help="only time the FP64 / int8 MFMA ground truth (top-100) of the 10M x 200 float32 and "
                          "10M x 128 uint8 workloads and exit (for matrix-core counter profiles)")
@@ -360,12 +364,18 @@ def main():
 
     configs = {}
     if not args.no_configs and args.workload == "mips":
-        for name, extra in (("s", per_query_beams(torch, max(1, args.steps // 4))), ("i8", None), ("u8", None)):
-            r, *rest = run_workload(torch, name, ["native"], max(1, args.steps // 2), 1, extra)
-            rest[3].close()
-            del rest
+        sub = [("s", per_query_beams(torch, max(1, args.steps // 4))), ("i8", None), ("u8", None)]
+        if args.cpu_configs:
+            sub.insert(0, ("xs", None))
+        for name, extra in sub:
+            r, Xc, Qc, wc, idxc, Gc, gic, gdc = run_workload(torch, name, ["native"], max(1, args.steps // 2), 1, extra)
+            idxc.close()
+            del idxc
             torch.cuda.empty_cache()
-            configs[{"s": "100Kx20_f32", "i8": "1Mx100_i8", "u8": "10Mx128_u8"}[name]] = r
+            if args.cpu_configs:
+                r["cpu_baseline"] = cpu_baseline(Xc, Qc, wc, Gc, gic, gdc, r["modes"]["native"]["Ls"], args.cpu_build_n)
+            del Xc, Qc, Gc
+            configs[{"xs": "10Kx20_f32", "s": "100Kx20_f32", "i8": "1Mx100_i8", "u8": "10Mx128_u8"}[name]] = r
 
     key = f"{args.workload} {best} Ls={top['Ls']}"
     traffic, traffic_src = profiled_traffic(key)
