@@ -307,9 +307,6 @@ struct Dist<float, METRIC> {
 // the number of chunk slots a lane walks (cps <= MAXC): instantiated for the
 // share itself, the chain adds nothing for slots past it.
 constexpr uint32_t kSplitChunks = 8;  // at most this many chunks per lane
-// float32 rows are whole 64-byte multiples (4 chunks), so a split over the
-// fewest lanes leaving at most kSplitChunks gives shares of 5 to 8 chunks
-constexpr uint32_t kSplitChunksMin = 5;
 
 template <int METRIC, uint32_t MAXC = kSplitChunks>
 __device__ __forceinline__ float split_row_dist(const float* row, const float* q, uint32_t ng, uint32_t S,
