@@ -38,7 +38,7 @@ enum BufId {
 static_assert(B_COUNT <= 48, "scratch slots");
 
 // Device words of B_FLAGS (u32 index): [0] search work counter, [1] visited
-// list overflow, [4] error status (bit 0 degree overflow, bit 1 prune scratch
+// list overflow, [2..3] the ordering pre-pass's work counter and status, [4] error status (bit 0 degree overflow, bit 1 prune scratch
 // exhausted), [8..9] pair count (u64), [10..11] group count (u64), [12]
 // largest group, [16] prune work counter, [17] long-list count, [18]
 // long-list work counter, [20..21] long-list scratch handed out (u64),
@@ -46,7 +46,8 @@ static_assert(B_COUNT <= 48, "scratch slots");
 // (u64), [26..27] dist_cmps of the build's insert searches (u64).
 constexpr size_t kFlagBytes = 256;
 enum FlagWord : uint32_t {
-  F_WORK = 0, F_OVERFLOW = 1, F_STATUS = 4, F_PAIRS = 8, F_GROUPS = 10, F_MAXG = 12, F_PWORK = 16, F_NBIG = 17,
+  F_WORK = 0, F_OVERFLOW = 1, F_PREWORK = 2, F_PRESTATUS = 3, F_STATUS = 4, F_PAIRS = 8, F_GROUPS = 10,
+  F_MAXG = 12, F_PWORK = 16, F_NBIG = 17,
   F_PWORK2 = 18, F_GUSED = 20, F_ROUNDB = 24, F_CMPS = 26
 };
 
@@ -1459,14 +1460,27 @@ struct SearchExtra {
 // longest expected first, so that the launch's end is not set by a long
 // query that started late.  With per-query beams the key is the beam (the
 // work grows with it): 100K x 20 with beams {10..200}, 1.70 -> 1.40 ms.
-// With DDG_ORDER=start, and more queries than resident waves, uniform-beam
-// launches are ordered by the distance to the start point (larger first), a
-// weak predictor of a query's length: 10M x 128 uint8 at Ls=140 3.45 ->
-// 3.20 ms per launch including the ordering, but 10M x 200 MIPS FP8 at
-// Ls=250 unchanged (3.52 ms) and native FP32 at Ls=275 slower (5.88 -> 5.98
-// ms); so it is not the default.  Keys sort ascending and stably, so equal keys keep the queries'
-// own order.  Results do not depend on the order.  Allocates: not for the
-// allocation-free path.  DDG_ORDER=0 keeps the queries in their own order.
+// With a uniform beam of at least kPreMinBeam and more queries than resident
+// waves, a pre-pass searches every query with beam kPreBeam for at most
+// kPreLimit visits (k = 1), and the launch takes the queries in descending
+// order of that search's best distance: a query whose neighborhood the short
+// search finds far away tends to need many expansions (Spearman 0.49 against
+// the visit count at Ls=250 on 10M x 200 MIPS FP8, where the distance to the
+// start point gives 0.05).  Wall time per call, pre-pass, keys and sort
+// included (main kernel alone in brackets), 10,000 queries: 10M x 200 MIPS
+// FP8 at Ls=250 3.54 -> 3.36 ms (3.49 -> 2.97), BF16 at Ls=275 4.21 -> 4.15
+// ms, FP32 at Ls=275 5.78 -> 5.80 ms; 10M x 128 uint8 at Ls=140 3.44 -> 3.31
+// ms (3.40 -> 2.96).  The pre-pass costs about 0.35 ms whatever the beam, so
+// shorter searches lose (FP8 at Ls=100 1.93 -> 2.05 ms, uint8 at Ls=100 2.67
+// -> 2.71 ms): hence the threshold.
+// DDG_ORDER=start orders uniform-beam launches by the distance to the start
+// point instead (larger first; 10M x 200 MIPS FP8 unchanged), DDG_ORDER=0
+// keeps the queries in their own order, DDG_ORDER_MIN_BEAM=<b> moves the
+// pre-pass threshold.  Keys sort ascending and stably, so equal keys keep the
+// queries' own order.  Results do not depend on the order.  Allocates: not
+// for the allocation-free path.
+constexpr uint32_t kPreBeam = 10, kPreLimit = 8, kPreMinBeam = 140;
+
 __global__ void k_beam_order_keys(const uint32_t* beams, uint32_t beam_max, uint64_t nq, uint32_t* keys,
                                   uint32_t* vals) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1476,13 +1490,70 @@ __global__ void k_beam_order_keys(const uint32_t* beams, uint32_t beam_max, uint
   vals[i] = (uint32_t)i;
 }
 
+// Keys of the pre-pass order: the best distance, larger first, as a 16-bit
+// key that sorts ascending (the top half of the complemented
+// order-preserving bits: sign, exponent and 7 mantissa bits; two sort
+// passes instead of four).
+__global__ void k_dist_order_keys(const float* d, uint64_t nq, uint32_t* keys, uint32_t* vals) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const uint32_t b = __float_as_uint(d[i]);
+  keys[i] = (~(b ^ ((uint32_t)((int32_t)b >> 31) | 0x80000000u))) >> 16;
+  vals[i] = (uint32_t)i;
+}
+
+// The pre-pass search of the uniform-beam order: beam kPreBeam, k = 1, at
+// most kPreLimit visits, its filter in LDS; best distances to dist (nq
+// floats), ids to ids.  Its work counter and status word are F_PREWORK and
+// F_PRESTATUS.  False when the short search's default shape would need a
+// global filter (the order is then not taken).
+static bool prepass_search(ddg_index* idx, int trav, const SearchArgs& a, uint32_t* ids, float* dist,
+                           hipError_t* err) {
+  SearchArgs p{};
+  Launch pl{};
+  *err = hipSuccess;
+  if (!search_layout_cfg(idx, kPreBeam, a.nstarts, trav, nullptr, p, pl) || p.gtable_words) return false;
+  p.base = a.base;
+  p.graph = a.graph;
+  p.queries = a.queries;
+  p.qids = a.qids;
+  p.starts = a.starts;
+  p.out_ids = ids;
+  p.out_dists = dist;
+  p.work = a.work - F_WORK + F_PREWORK;
+  p.overflow = a.work - F_WORK + F_PRESTATUS;
+  p.n = a.n;
+  p.nq = a.nq;
+  p.limit = kPreLimit;
+  p.cut = a.cut;
+  p.d = a.d;
+  p.ad = a.ad;
+  p.R = a.R;
+  p.nstarts = a.nstarts;
+  p.k = 1;
+  p.beam = kPreBeam;
+  p.degree_limit = a.degree_limit;
+  p.build_mode = 0;
+  p.cut_trim = a.cut_trim;
+  p.rerank = 1;
+  pl.blocks = (uint32_t)std::min<uint64_t>(pl.blocks, (a.nq + pl.waves_per_block - 1) / pl.waves_per_block);
+  *err = launch_fill_u32(p.work, 0u, 2, idx->stream);
+  if (*err == hipSuccess) *err = launch_search(idx->dtype, idx->metric, trav, p, pl, idx->stream);
+  return *err == hipSuccess;
+}
+
 static ddg_status order_queries(ddg_index* idx, int trav, const uint32_t* beams, uint32_t beam_max, uint64_t nq,
                                 const Launch& l, SearchArgs& a) {
   a.order = nullptr;
   const char* e = getenv("DDG_ORDER");
   if (nq < 2 || (e && e[0] == '0')) return DDG_OK;
   const bool by_start = e && strcmp(e, "start") == 0;
-  if (!beams && (!by_start || nq <= (uint64_t)l.blocks * l.waves_per_block)) return DDG_OK;
+  const bool more = nq > (uint64_t)l.blocks * l.waves_per_block;  // queries beyond the resident waves
+  uint32_t min_beam = kPreMinBeam;
+  const char* mb = getenv("DDG_ORDER_MIN_BEAM");
+  if (mb && atoi(mb) > 0) min_beam = (uint32_t)atoi(mb);
+  const bool by_pre = !beams && !by_start && more && a.beam >= min_beam && a.beam > kPreBeam;
+  if (!beams && !(by_start && more) && !by_pre) return DDG_OK;
   ENSURE(B_OKEYS, nq * 4);
   ENSURE(B_OVALS, nq * 4);
   ENSURE(B_OKEYS2, nq * 4);
@@ -1495,6 +1566,17 @@ static ddg_status order_queries(ddg_index* idx, int trav, const uint32_t* beams,
     HIPCHK(hipGetLastError());
     kb = 1;
     while (kb < 32 && (1ull << kb) <= beam_max) kb++;
+  } else if (by_pre) {
+    // the pre-pass writes into the sort's second buffers, which the sort
+    // overwrites only after the keys are made from them
+    hipError_t err;
+    if (!prepass_search(idx, trav, a, buf<uint32_t>(idx, B_OKEYS2), buf<float>(idx, B_OVALS2), &err)) {
+      HIPCHK(err);
+      return DDG_OK;
+    }
+    hipLaunchKernelGGL(k_dist_order_keys, dim3((uint32_t)((nq + 255) / 256)), dim3(256), 0, idx->stream,
+                       buf<float>(idx, B_OVALS2), nq, buf<uint32_t>(idx, B_OKEYS), buf<uint32_t>(idx, B_OVALS));
+    HIPCHK(hipGetLastError());
   } else {
     SearchArgs k = a;
     k.lds_per_wave = trav == DDG_TRAV_FP8E4M3 ? 2 * a.qrow_bytes : a.qrow_bytes;

@@ -108,6 +108,16 @@ __device__ __forceinline__ uint32_t count_not_above(const uint32_t* U, uint32_t 
   return lo;
 }
 
+// Lanes of a prefetched adjacency row whose filter words are loaded ahead:
+// those the next expansion's neighbor filter reads (below the row's degree
+// and the degree limit).  Words past the degree hold stale ids from earlier
+// rounds of the build; their random slots would each cost a line of the
+// filter slice for nothing.
+__device__ __forceinline__ uint32_t pf_m(const SearchArgs& a, uint32_t deg) {
+  const uint32_t m = deg < a.degree_limit ? deg : a.degree_limit;
+  return m < a.R ? m : a.R;
+}
+
 // STAGE: candidate rows are gathered into LDS before their distances (a
 // launch choice for long rows; a separate instantiation, so the other
 // kernels do not carry its registers).  The native kernels are compiled for
@@ -542,7 +552,7 @@ void k_search(SearchArgs a) {
         if (!cand_next && pf_id != 0xFFFFFFFFu) {
           wave_sync_global();  // this expansion's table stores by other lanes
           pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
-          pf_prev = (lane < a.R && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u))) ? table[pf_slot]
+          pf_prev = (lane < pf_m(a, pf_deg) && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u))) ? table[pf_slot]
                                                                                          : 0xFFFFFFFFu;
           pf_table = true;
         }
@@ -628,7 +638,7 @@ void k_search(SearchArgs a) {
         if (cand_next) {  // a new candidate is the prediction: its row was loaded during the merge
           wave_sync_global();
           pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
-          pf_prev = (lane < a.R && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u))) ? table[pf_slot]
+          pf_prev = (lane < pf_m(a, pf_deg) && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u))) ? table[pf_slot]
                                                                                          : 0xFFFFFFFFu;
           pf_table = true;
         }

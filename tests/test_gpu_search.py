@@ -165,20 +165,20 @@ def test_search_per_query_beam(oracle, xs):
     assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
 
 
-def _small_case(oracle, dtype, metric, n, d, seed, R=24, L=40, alpha=1.2):
+def _small_case(oracle, dtype, metric, n, d, seed, R=24, L=40, alpha=1.2, nq=300):
     from distributed_diskann_amd.formats import align_rows
     rng = np.random.default_rng(seed)
     if dtype == "uint8":
         C = rng.uniform(0, 200, (20, d))
         X = np.clip(np.rint(C[rng.integers(0, 20, n)] + rng.normal(0, 15, (n, d))), 0, 255).astype(np.uint8)
-        Q = np.clip(np.rint(C[rng.integers(0, 20, 300)] + rng.normal(0, 15, (300, d))), 0, 255).astype(np.uint8)
+        Q = np.clip(np.rint(C[rng.integers(0, 20, nq)] + rng.normal(0, 15, (nq, d))), 0, 255).astype(np.uint8)
     elif dtype == "int8":
         C = rng.normal(0, 30, (20, d))
         X = np.clip(np.rint(C[rng.integers(0, 20, n)] + rng.normal(0, 12, (n, d))), -128, 127).astype(np.int8)
-        Q = np.clip(np.rint(C[rng.integers(0, 20, 300)] + rng.normal(0, 12, (300, d))), -128, 127).astype(np.int8)
+        Q = np.clip(np.rint(C[rng.integers(0, 20, nq)] + rng.normal(0, 12, (nq, d))), -128, 127).astype(np.int8)
     else:
         X = rng.standard_normal((n, d)).astype(np.float32)
-        Q = rng.standard_normal((300, d)).astype(np.float32)
+        Q = rng.standard_normal((nq, d)).astype(np.float32)
     Xa, Qa = align_rows(X), align_rows(Q)
     P = oracle.Points(Xa, d, metric)
     G = oracle.build(P, R, L, alpha, tie_mode=oracle.TIE_ID)
@@ -335,15 +335,21 @@ def test_ground_truth_vs_oracle(oracle, xs):
         assert np.array_equal(ids, oi) and np.array_equal(d, od)
 
 
-@pytest.mark.parametrize("order", ["start", "0"])
+@pytest.mark.parametrize("order", ["pre", "start", "0"])
 def test_search_query_order(oracle, monkeypatch, order):
-    """The order a launch takes its queries in (largest beam first by
-    default; by distance to the start point with DDG_ORDER=start; none with
-    DDG_ORDER=0) changes no result: 12,000 queries (more than the resident
-    waves) bit-exact against the oracle, with uniform and per-query beams."""
+    """The order a launch takes its queries in (by default the largest beam
+    first, and for a long uniform beam, here from 100 on, the descending best
+    distance of a short pre-pass search; by distance to the start point with
+    DDG_ORDER=start; none with DDG_ORDER=0) changes no result: 12,000 queries
+    (more than the resident waves) bit-exact against the oracle, with uniform
+    and per-query beams."""
     from distributed_diskann_amd.formats import align_rows
     print(f"[gpu] query order {order}", flush=True)
-    monkeypatch.setenv("DDG_ORDER", order)
+    if order == "pre":
+        monkeypatch.delenv("DDG_ORDER", raising=False)
+        monkeypatch.setenv("DDG_ORDER_MIN_BEAM", "100")
+    else:
+        monkeypatch.setenv("DDG_ORDER", order)
     rng = np.random.default_rng(77)
     d = 20
     X = align_rows(rng.standard_normal((20000, d)).astype(np.float32))
@@ -358,4 +364,44 @@ def test_search_query_order(oracle, monkeypatch, order):
         assert np.array_equal(ids, oi)
         assert np.array_equal(dists.view(np.uint32), od.view(np.uint32))
         assert np.array_equal(cmps, oc) and np.array_equal(vis, ov)
+    idx.close()
+
+
+@pytest.mark.parametrize("dtype,metric,trav", [("float32", "mips", "fp8"), ("float32", "l2", "bf16"),
+                                               ("uint8", "l2", "native"), ("float32", "mips", "native")])
+def test_search_prepass_order_identical(oracle, monkeypatch, dtype, metric, trav):
+    """Uniform-beam launches with more queries than resident waves take their
+    queries in the order of a short pre-pass search (beam 10, k = 1, 8 visits,
+    re-ranking its best entry in the reduced modes) when the beam is 140 or
+    more; every output equals the launch in the queries' own order
+    (DDG_ORDER=0), at beams above and below that threshold."""
+    print(f"[gpu] pre-pass order {dtype} {metric} {trav}", flush=True)
+    torch = pytest.importorskip("torch")
+    d = 48
+    X, Q, P, G = _small_case(oracle, dtype, metric, 10000, d, seed=11, nq=12000)
+    idx = _index(X, d, G.rows, metric)
+    if trav != "native":
+        idx.prepare_traversal(trav)
+    dev = torch.device("cuda:0")
+    Qd = torch.from_numpy(Q).to(dev)
+    nq = Q.shape[0]
+    starts = torch.zeros(1, dtype=torch.int32, device=dev)
+    rr = 100 if trav != "native" else 0
+
+    def run(Ls):
+        ids = torch.empty((nq, 10), dtype=torch.int32, device=dev)
+        dd = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+        cm = torch.empty(nq, dtype=torch.int64, device=dev)
+        vs = torch.empty(nq, dtype=torch.int32, device=dev)
+        idx.search_device(Qd, 10, Ls, ids, dd, cm, vs, cut=1.35, starts=starts, trav=trav, rerank=rr)
+        torch.cuda.synchronize()
+        return [t.cpu().numpy() for t in (ids, dd.view(torch.int32), cm, vs)]
+
+    for Ls in (50, 150):
+        monkeypatch.delenv("DDG_ORDER", raising=False)
+        got = run(Ls)
+        monkeypatch.setenv("DDG_ORDER", "0")
+        want = run(Ls)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
     idx.close()
