@@ -120,7 +120,11 @@ ddg_status ddg_robust_prune(ddg_index* idx, uint64_t np, const uint32_t* pts, co
  * beamSearch.h:128 (NULL = the query's row index, as searchAll does).
  * per_query_beam (nullable) gives each query its own beam width.  Outputs
  * frontier[0:k] ids and distances, dist_cmps and |visited| per query
- * (nullable). */
+ * (nullable).  The launch takes its queries longest expected first: by beam
+ * when beams differ, and, for a uniform beam of 140 or more with more queries
+ * than resident wavefronts, by the best distance of a short pre-pass search
+ * (beam 10, at most 8 visits) run in the same call.  Outputs do not depend
+ * on that order; environment DDG_ORDER=0 turns it off. */
 ddg_status ddg_search(ddg_index* idx, const void* queries /* nq*aligned_d */, uint64_t nq,
                       const uint32_t* query_ids, const uint32_t* starts, uint32_t nstarts,
                       const ddg_query_params* qp, const uint32_t* per_query_beam, uint32_t* out_ids,
