@@ -404,4 +404,30 @@ def test_search_prepass_order_identical(oracle, monkeypatch, dtype, metric, trav
         want = run(Ls)
         for g, w in zip(got, want):
             assert np.array_equal(g, w)
+    if trav == "native" and dtype == "uint8":
+        # three start points and caller-given query ids (the self-skip id),
+        # which the pre-pass takes too; the host-buffer entry point as well
+        starts = torch.tensor([0, 17, 4242], dtype=torch.int32, device=dev)
+        qids = np.random.default_rng(5).integers(0, 10000, nq).astype(np.uint32)
+        Qid = torch.from_numpy(qids.view(np.int32)).to(dev)
+
+        def run_ids(Ls):
+            ids = torch.empty((nq, 10), dtype=torch.int32, device=dev)
+            dd = torch.empty((nq, 10), dtype=torch.float32, device=dev)
+            cm = torch.empty(nq, dtype=torch.int64, device=dev)
+            vs = torch.empty(nq, dtype=torch.int32, device=dev)
+            idx.search_device(Qd, 10, Ls, ids, dd, cm, vs, cut=1.35, starts=starts, query_ids=Qid)
+            torch.cuda.synchronize()
+            return [t.cpu().numpy() for t in (ids, dd.view(torch.int32), cm, vs)]
+
+        monkeypatch.delenv("DDG_ORDER", raising=False)
+        got = run_ids(150)
+        host = idx.search(Q, 10, 150, 1.35, starts=np.array([0, 17, 4242], np.uint32), query_ids=qids)
+        monkeypatch.setenv("DDG_ORDER", "0")
+        want = run_ids(150)
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w)
+        assert np.array_equal(host[0].view(np.uint32), want[0].view(np.uint32))
+        assert np.array_equal(host[1].view(np.uint32), want[1].view(np.uint32))
+        assert np.array_equal(host[2], want[2]) and np.array_equal(host[3], want[3])
     idx.close()
