@@ -397,6 +397,10 @@ def main():
         "fp32": None if fp32 is None else {"qps": fp32["qps"], "Ls": fp32["Ls"], "recall": fp32["recall"]},
         "speedup_vs_fp32": None if fp32 is None else top["qps"] / fp32["qps"],
         "host_buffers": host,
+        "timing": ("value and ms_per_step: whole search calls, wall clock, inputs and outputs in device memory; "
+                   "from a beam of 140 on a call also runs a short pre-pass search (beam 10, 8 visits) and a "
+                   "key sort that order its queries (about 0.35 ms). kernel_ms and roofline: the main search "
+                   "launch alone, HIP events on its stream"),
         "headline": head,
         "configs": configs,
         "roofline": {"bound": "hbm", "achieved": top["alg_GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
