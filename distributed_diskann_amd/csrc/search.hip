@@ -552,8 +552,9 @@ void k_search(SearchArgs a) {
         if (!cand_next && pf_id != 0xFFFFFFFFu) {
           wave_sync_global();  // this expansion's table stores by other lanes
           pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
-          pf_prev = (lane < pf_m(a, pf_deg) && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u))) ? table[pf_slot]
-                                                                                         : 0xFFFFFFFFu;
+          pf_prev = (lane < pf_m(a, pf_deg) && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u)))
+                        ? table[pf_slot]
+                        : 0xFFFFFFFFu;
           pf_table = true;
         }
       }
@@ -638,8 +639,9 @@ void k_search(SearchArgs a) {
         if (cand_next) {  // a new candidate is the prediction: its row was loaded during the merge
           wave_sync_global();
           pf_slot = (uint32_t)hash64_2((uint64_t)pf_nb) & mask;
-          pf_prev = (lane < pf_m(a, pf_deg) && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u))) ? table[pf_slot]
-                                                                                         : 0xFFFFFFFFu;
+          pf_prev = (lane < pf_m(a, pf_deg) && (!WB || ((wb[pf_slot >> 5] >> (pf_slot & 31u)) & 1u)))
+                        ? table[pf_slot]
+                        : 0xFFFFFFFFu;
           pf_table = true;
         }
       }
