@@ -1471,8 +1471,8 @@ struct SearchExtra {
 // FP8 at Ls=250 3.54 -> 3.36 ms (3.49 -> 2.97), BF16 at Ls=275 4.21 -> 4.15
 // ms, FP32 at Ls=275 5.78 -> 5.80 ms; 10M x 128 uint8 at Ls=140 3.44 -> 3.31
 // ms (3.40 -> 2.96).  Pre-pass, keys and sort cost about 0.35 ms whatever
-// the beam, so shorter searches lose (FP8 at Ls=100 1.93 -> 2.05 ms, uint8 at Ls=100 2.67
-// -> 2.71 ms): hence the threshold.
+// the beam, so shorter searches lose (FP8 at Ls=100 1.93 -> 2.05 ms, uint8
+// at Ls=100 2.67 -> 2.71 ms): hence the threshold.
 // DDG_ORDER=start orders uniform-beam launches by the distance to the start
 // point instead (larger first; 10M x 200 MIPS FP8 unchanged), DDG_ORDER=0
 // keeps the queries in their own order, DDG_ORDER_MIN_BEAM=<b> moves the
