@@ -1466,7 +1466,7 @@ struct SearchExtra {
 // order of that search's best distance: a query whose neighborhood the short
 // search finds far away tends to need many expansions (Spearman 0.49 against
 // the visit count at Ls=250 on 10M x 200 MIPS FP8, where the distance to the
-// start point gives 0.05).  Wall time per call, pre-pass, keys and sort
+// start point gives 0.18).  Wall time per call, pre-pass, keys and sort
 // included (main kernel alone in brackets), 10,000 queries: 10M x 200 MIPS
 // FP8 at Ls=250 3.54 -> 3.36 ms (3.49 -> 2.97), BF16 at Ls=275 4.21 -> 4.15
 // ms, FP32 at Ls=275 5.78 -> 5.80 ms; 10M x 128 uint8 at Ls=140 3.44 -> 3.31
